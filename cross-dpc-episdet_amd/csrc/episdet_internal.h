@@ -1,0 +1,111 @@
+// episdet_internal.h -- structures shared by the HIP kernels and the host
+// library.  Not part of the public ABI (include/episdet.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace epi {
+
+constexpr int kLanes = 64;     // CDNA wavefront
+constexpr int kTop = 64;       // candidates kept per cohort
+constexpr int kWaveCap = 128;  // per-wavefront LDS candidate buffer
+
+// Candidate: FP32 score and a lexicographic key (i << 42 | j << 21 | k).
+struct Cand {
+    float s;
+    uint32_t pad;
+    uint64_t key;
+};
+
+__host__ __device__ inline uint64_t make_key(uint32_t i, uint32_t j, uint32_t k)
+{
+    return ((uint64_t)i << 42) | ((uint64_t)j << 21) | (uint64_t)k;
+}
+__host__ __device__ inline uint32_t key_i(uint64_t key) { return (uint32_t)(key >> 42); }
+__host__ __device__ inline uint32_t key_j(uint64_t key) { return (uint32_t)((key >> 21) & 0x1FFFFF); }
+__host__ __device__ inline uint32_t key_k(uint64_t key) { return (uint32_t)(key & 0x1FFFFF); }
+constexpr uint64_t kEmptyKey = ~0ull;
+
+// Resident device layout of one cohort.  Class 0 = controls, 1 = cases.
+// A 64-bit sample word of one SNP is stored as uint4 {g0.lo, g0.hi, g1.lo,
+// g1.hi}; sample t of a class sits at bit t % 64 of word t / 64, padding
+// bits are zero in both planes.
+struct CohortDev {
+    const uint4 *sm[2];   // SNP-major  [Mp][W[c]]   (uniform loads)
+    const uint4 *wm[2];   // word-major [W[c]][Mp]   (per-lane loads)
+    uint4 *pair[2];       // [Mp][Mp] for p < q: {n00, n01, n10, n11}
+    uint2 *single[2];     // [Mp]: {count g0, count g1}
+    uint32_t W[2];        // 64-bit words per class
+    uint32_t n[2];        // class sizes
+    uint32_t M, Mp;       // SNPs, padded to a multiple of 64
+    uint32_t pad[2];
+};
+
+// One (cohort, k-block, j-group) row of the triplet-tile queue.  Items of
+// the row are consecutive chunks of `waves` i values, i in [0, icount).
+struct Row {
+    uint32_t cohort;
+    uint32_t kb;          // k in [64 kb, 64 kb + 64)
+    uint32_t jg;          // j in [snps_j * jg, snps_j * jg + snps_j)
+    uint32_t icount;
+    uint32_t item_start;  // first queue item of this row
+    uint32_t pad;
+};
+
+struct ScanParams {
+    const CohortDev *cohorts;
+    const Row *rows;
+    uint32_t n_rows;
+    uint32_t n_items;
+    uint32_t *counters;     // [0] item queue head, [1 + c] slot count of cohort c
+    const float *table;     // ln(n!) FP32, table_len entries
+    uint32_t table_len;
+    uint32_t slot_cap;      // slots per cohort
+    Cand *slots;            // [cohort][slot_cap][kTop]
+    uint32_t *overflow;     // set to 1 if a slot could not be stored
+    float *dump_scores;     // dump mode: score per triplet rank (cohort 0)
+    uint32_t *dump_counts;  // dump mode: 54 counts per triplet rank
+};
+
+// Rescoring output for one candidate.
+struct Rescored {
+    uint32_t i, j, k, valid;
+    float s32;
+    float s32_scan;
+    double s64;
+    uint32_t counts[54];
+    uint32_t pad[2];
+};
+
+// Scan kernel configuration (template instance selector).
+struct ScanConfig {
+    int snps_j;  // 1, 2, 4, 8
+    int waves;   // 4, 8, 16
+    int words;   // 1 or 2 sample words per loop step
+};
+
+// Launch wrappers (episdet_kernels.hip).
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
+                       const uint32_t *order, uint32_t n0, uint32_t n1,
+                       uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1,
+                       uint32_t W0, uint32_t W1, hipStream_t s);
+hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &host_cohort,
+                        hipStream_t s);
+bool scan_config_supported(const ScanConfig &cfg);
+hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump,
+                          size_t lds_bytes, int *blocks_per_cu);
+size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len);
+hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes);
+hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump,
+                       const ScanParams &p, uint32_t grid, hipStream_t s);
+hipError_t launch_merge(const Cand *slots, const uint32_t *counters,
+                        uint32_t slot_cap, uint32_t n_cohorts, Cand *merged,
+                        hipStream_t s);
+hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands,
+                          uint32_t per_cohort, uint32_t n_cohorts,
+                          const float *t32, const double *t64, Rescored *out,
+                          hipStream_t s);
+
+}  // namespace epi

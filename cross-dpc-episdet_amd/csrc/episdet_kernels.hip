@@ -1,0 +1,710 @@
+// episdet_kernels.hip -- gfx950 kernels of the exhaustive third-order
+// epistasis scan with K2 scoring.
+//
+// Reference: the DPC++ kernel lambda of GPU/v4.cpp:297-593 (one work-item
+// per triplet, 27 three-way ANDs + 27 popcounts per 32-bit word and class,
+// K2 from a log-factorial table, per-slot running minimum).  This file
+// redesigns it for CDNA4:
+//   * pack_kernel    class split + bit-packing into 64-bit planes, two
+//                    layouts (SNP-major for scalar loads, word-major for
+//                    coalesced per-lane loads)          (GPU/v4.cpp:112-196)
+//   * pairs_kernel   pair and single margins of the {0,1} genotype planes
+//   * scan_kernel    persistent; pulls (cohort, k-block, j-group, i-chunk)
+//                    items from a device queue.  A wavefront owns one i, a
+//                    group of snps_j consecutive j and 64 consecutive k (one
+//                    per lane).  Pair products P_ab = g_a(i) & g_b(j) are
+//                    wave-uniform and live in SGPRs (SALU work); each lane
+//                    then issues v_and + v_bcnt for the 8 cells with genotypes
+//                    in {0,1}.  The 19 cells involving genotype 2 are exact
+//                    differences of pair/single margins (episdet_cells.h).
+//                    K2 from the ln(n!) table (LDS-resident when it fits),
+//                    then a ballot-filtered per-wave top-64 in LDS.
+//                                                       (GPU/v4.cpp:283-597)
+//   * merge_kernel   per-cohort top-64 over all workgroup slots
+//                                                       (GPU/v4.cpp:598-608)
+//   * rescore_kernel recounts each candidate with all 27 cells explicitly
+//                    (~(g0|g1) under the padding mask) and scores it in FP32
+//                    and FP64.
+#include <math.h>
+
+#include "episdet_cells.h"
+#include "episdet_internal.h"
+
+#define EPI_CONST __attribute__((address_space(4)))
+
+namespace epi {
+
+template <typename T>
+__device__ __forceinline__ const EPI_CONST T *as_const(const T *p)
+{
+    return (const EPI_CONST T *)p;
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Loads through the constant address space: with a wave-uniform address the
+// compiler issues them as scalar (SMEM) loads into SGPRs.
+__device__ __forceinline__ uint4 ldc4(const uint4 *p)
+{
+    const EPI_CONST uint32_t *q = (const EPI_CONST uint32_t *)p;
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ uint2 ldc2(const uint2 *p)
+{
+    const EPI_CONST uint32_t *q = (const EPI_CONST uint32_t *)p;
+    return make_uint2(q[0], q[1]);
+}
+
+__device__ __forceinline__ bool cand_less(float s, uint64_t k, float ts, uint64_t tk)
+{
+    return s < ts || (s == ts && k < tk);
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint64_t c3(uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; }
+__device__ __forceinline__ uint64_t c2(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
+
+// lexicographic rank of i < j < k among C(M,3)
+__device__ __forceinline__ uint64_t rank3(uint32_t M, uint32_t i, uint32_t j, uint32_t k)
+{
+    return c3(M) - c3(M - i) + c2(M - i - 1) - c2(M - j) + (uint64_t)(k - j - 1);
+}
+
+// ------------------------------------------------------------------------
+// small utilities
+// ------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// packing (GPU/v4.cpp:112-196): order[] lists controls then cases, each in
+// original sample order; sample t of class c goes to bit t%64 of word t/64.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+pack_kernel(const uint8_t *__restrict__ geno, uint32_t M, uint32_t Mp,
+            const uint32_t *__restrict__ order, uint32_t n0, uint32_t n1,
+            uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1)
+{
+    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t w = blockIdx.y;
+    const uint32_t c = blockIdx.z;
+    const uint32_t W = c ? W1 : W0;
+    if (s >= Mp || w >= W)
+        return;
+    const uint32_t n = c ? n1 : n0;
+    const uint32_t *ord = order + (c ? n0 : 0);
+    uint64_t g0 = 0, g1 = 0;
+    if (s < M) {
+        const uint32_t t0 = w * 64;
+        const uint32_t tn = (n - t0) < 64u ? (n - t0) : 64u;
+        for (uint32_t t = 0; t < tn; ++t) {
+            const uint8_t g = geno[(size_t)ord[t0 + t] * M + s];
+            g0 |= (uint64_t)(g == 0) << t;
+            g1 |= (uint64_t)(g == 1) << t;
+        }
+    }
+    const uint4 v = make_uint4((uint32_t)g0, (uint32_t)(g0 >> 32), (uint32_t)g1,
+                               (uint32_t)(g1 >> 32));
+    (c ? sm1 : sm0)[(size_t)s * W + w] = v;
+    (c ? wm1 : wm0)[(size_t)w * Mp + s] = v;
+}
+
+hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
+                       const uint32_t *order, uint32_t n0, uint32_t n1, uint4 *sm0,
+                       uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1,
+                       hipStream_t s)
+{
+    (void)N;
+    const uint32_t wmax = W0 > W1 ? W0 : W1;
+    if (wmax == 0)
+        return hipSuccess;
+    dim3 grid((Mp + 255) / 256, wmax, 2);
+    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, geno, M, Mp, order, n0, n1,
+                       sm0, wm0, sm1, wm1, W0, W1);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// pair and single margins
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict__ cd)
+{
+    const uint32_t c = blockIdx.z;
+    const uint32_t p = blockIdx.y;
+    const uint32_t qb = blockIdx.x * 256;
+    if (qb + 255 < p)
+        return;
+    const uint32_t q = qb + threadIdx.x;
+    const uint32_t Mp = as_const(cd)->Mp;
+    if (q >= Mp)
+        return;
+    const uint32_t W = as_const(cd)->W[c];
+    const uint4 *X = as_const(cd)->sm[c] + (size_t)p * W;
+    const uint4 *Z = as_const(cd)->wm[c] + q;
+    uint32_t a00 = 0, a01 = 0, a10 = 0, a11 = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint4 x = ldc4(X + w);
+        const uint4 z = Z[(size_t)w * Mp];
+        a00 += __popc(x.x & z.x) + __popc(x.y & z.y);
+        a01 += __popc(x.x & z.z) + __popc(x.y & z.w);
+        a10 += __popc(x.z & z.x) + __popc(x.w & z.y);
+        a11 += __popc(x.z & z.z) + __popc(x.w & z.w);
+    }
+    if (q > p)
+        as_const(cd)->pair[c][(size_t)p * Mp + q] = make_uint4(a00, a01, a10, a11);
+    else if (q == p)
+        as_const(cd)->single[c][p] = make_uint2(a00, a11);
+}
+
+hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &h, hipStream_t s)
+{
+    dim3 grid((h.Mp + 255) / 256, h.Mp, 2);
+    hipLaunchKernelGGL(pairs_kernel, grid, dim3(256), 0, s, dev_cohort);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// scan kernel
+// ------------------------------------------------------------------------
+// 8 cells (a, b, c in {0,1}) of JP triplets (i, j0 + jj, k) over one class.
+// x (SNP i) and y (SNP j) are wave-uniform: their loads are scalar and the
+// pair products are SALU work.  z (SNP k = lane) is a per-lane uint4.
+template <int JP>
+__device__ __forceinline__ void count8(const uint4 *X, const uint4 *Y,
+                                       uint32_t W, const uint4 *__restrict__ Z, uint32_t Mp,
+                                       uint32_t (&acc)[JP][8])
+{
+#pragma unroll
+    for (int jj = 0; jj < JP; ++jj)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            acc[jj][e] = 0;
+#pragma unroll 2
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint4 x = ldc4(X + w);
+        const uint4 z = Z[(size_t)w * Mp];
+#pragma unroll
+        for (int jj = 0; jj < JP; ++jj) {
+            const uint4 y = ldc4(Y + (size_t)jj * W + w);
+            const uint32_t p00l = x.x & y.x, p00h = x.y & y.y;  // g0(i) & g0(j)
+            const uint32_t p01l = x.x & y.z, p01h = x.y & y.w;  // g0(i) & g1(j)
+            const uint32_t p10l = x.z & y.x, p10h = x.w & y.y;  // g1(i) & g0(j)
+            const uint32_t p11l = x.z & y.z, p11h = x.w & y.w;  // g1(i) & g1(j)
+            acc[jj][0] += __popc(p00l & z.x) + __popc(p00h & z.y);
+            acc[jj][1] += __popc(p00l & z.z) + __popc(p00h & z.w);
+            acc[jj][2] += __popc(p01l & z.x) + __popc(p01h & z.y);
+            acc[jj][3] += __popc(p01l & z.z) + __popc(p01h & z.w);
+            acc[jj][4] += __popc(p10l & z.x) + __popc(p10h & z.y);
+            acc[jj][5] += __popc(p10l & z.z) + __popc(p10h & z.w);
+            acc[jj][6] += __popc(p11l & z.x) + __popc(p11h & z.y);
+            acc[jj][7] += __popc(p11l & z.z) + __popc(p11h & z.w);
+        }
+    }
+}
+
+// Wave-level top-kTop: rank every buffered candidate against the others
+// (keys are unique) and keep the kTop best, sorted, in buf[0..kTop).
+__device__ __noinline__ void wave_compact(Cand *buf, Cand *keep, int &cnt, float &ts,
+                                          uint64_t &tk, uint32_t lane)
+{
+    wave_sync();
+    const int n = cnt;
+    for (int e = (int)lane; e < n; e += kLanes) {
+        const Cand me = buf[e];
+        int rank = 0;
+        for (int f = 0; f < n; ++f) {
+            const Cand o = buf[f];
+            rank += cand_less(o.s, o.key, me.s, me.key) ? 1 : 0;
+        }
+        if (rank < kTop)
+            keep[rank] = me;
+    }
+    wave_sync();
+    const int m = n < kTop ? n : kTop;
+    if ((int)lane < m)
+        buf[lane] = keep[lane];
+    cnt = m;
+    if (m == kTop) {
+        ts = keep[kTop - 1].s;
+        tk = keep[kTop - 1].key;
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void wave_offer(bool valid, float s, uint64_t key, Cand *buf,
+                                           Cand *keep, int &cnt, float &ts, uint64_t &tk,
+                                           uint32_t lane)
+{
+    const bool pass = valid && cand_less(s, key, ts, tk);
+    const uint64_t mask = __ballot(pass);
+    if (mask == 0)
+        return;
+    const int nn = __popcll(mask);
+    if (cnt + nn > kWaveCap)
+        wave_compact(buf, keep, cnt, ts, tk, lane);
+    if (pass) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        Cand c;
+        c.s = s;
+        c.pad = 0;
+        c.key = key;
+        buf[cnt + below] = c;
+    }
+    cnt += nn;
+}
+
+template <int NW>
+struct ScanLds {
+    Cand cand[NW][kWaveCap + kTop];
+    uint32_t misc[4];
+    uint32_t wcnt[NW];
+};
+
+template <int NW>
+__host__ __device__ constexpr size_t scan_lds_fixed()
+{
+    return (sizeof(ScanLds<NW>) + 15) & ~(size_t)15;
+}
+
+// Flush the workgroup's candidates for `cohort` into one slot of kTop
+// sorted entries.  Called by every thread of the workgroup.
+template <int NW>
+__device__ void wg_flush(const ScanParams &P, ScanLds<NW> &L, uint32_t cohort, int &cnt,
+                         float &ts, uint64_t &tk, uint32_t wave, uint32_t lane)
+{
+    Cand *buf = L.cand[wave];
+    Cand *keep = buf + kWaveCap;
+    if (cnt > kTop)
+        wave_compact(buf, keep, cnt, ts, tk, lane);
+    if (lane == 0)
+        L.wcnt[wave] = (uint32_t)cnt;
+    if (threadIdx.x == 0)
+        L.misc[1] = atomicAdd(&P.counters[1 + cohort], 1u);
+    __syncthreads();
+    const uint32_t slot = L.misc[1];
+    const uint32_t mine = L.wcnt[wave];
+    const bool have = lane < mine;
+    Cand me;
+    me.s = INFINITY;
+    me.pad = 0;
+    me.key = kEmptyKey;
+    if (have)
+        me = buf[lane];
+    uint32_t rank = 0, total = 0;
+    for (int w2 = 0; w2 < NW; ++w2) {
+        const uint32_t n2 = L.wcnt[w2];
+        total += n2;
+        if (have)
+            for (uint32_t f = 0; f < n2; ++f) {
+                const Cand o = L.cand[w2][f];
+                rank += cand_less(o.s, o.key, me.s, me.key) ? 1u : 0u;
+            }
+    }
+    if (slot < P.slot_cap) {
+        Cand *out = P.slots + ((size_t)cohort * P.slot_cap + slot) * kTop;
+        if (have && rank < (uint32_t)kTop)
+            out[rank] = me;
+        if (threadIdx.x < (uint32_t)kTop && threadIdx.x >= total) {
+            Cand e;
+            e.s = INFINITY;
+            e.pad = 0;
+            e.key = kEmptyKey;
+            out[threadIdx.x] = e;
+        }
+    } else if (threadIdx.x == 0) {
+        atomicExch(P.overflow, 1u);
+    }
+    __syncthreads();
+    cnt = 0;
+    ts = INFINITY;
+    tk = kEmptyKey;
+}
+
+template <int JP, int NW, bool TLDS, bool DUMP>
+// At least 4 wavefronts per SIMD (<= 128 VGPRs) to hide load latency.
+__global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    ScanLds<NW> &L = *reinterpret_cast<ScanLds<NW> *>(lds_raw);
+    float *Tl = reinterpret_cast<float *>(lds_raw + scan_lds_fixed<NW>());
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = rfl(tid >> 6);
+    Cand *buf = L.cand[wave];
+    Cand *keep = buf + kWaveCap;
+
+    if (TLDS)
+        for (uint32_t t = tid; t < P.table_len; t += NW * 64)
+            Tl[t] = P.table[t];
+
+    int cnt = 0;
+    float ts = INFINITY;
+    uint64_t tk = kEmptyKey;
+    uint32_t cur = 0xFFFFFFFFu;
+    uint32_t r = 0;
+    const EPI_CONST Row *rows = as_const(P.rows);
+
+    for (;;) {
+        if (tid == 0)
+            L.misc[0] = atomicAdd(&P.counters[0], 1u);
+        __syncthreads();
+        const uint32_t item = rfl(L.misc[0]);
+        __syncthreads();
+        uint32_t cohort = 0xFFFFFFFFu;
+        if (item < P.n_items) {
+            while (r + 1 < P.n_rows && rows[r + 1].item_start <= item)
+                ++r;
+            cohort = rows[r].cohort;
+        }
+        if (cohort != cur) {
+            if (!DUMP && cur != 0xFFFFFFFFu)
+                wg_flush<NW>(P, L, cur, cnt, ts, tk, wave, lane);
+            cur = cohort;
+        }
+        if (item >= P.n_items)
+            break;
+
+        const uint32_t kb = rows[r].kb;
+        const uint32_t jg = rows[r].jg;
+        const uint32_t icount = rows[r].icount;
+        const uint32_t i = (item - rows[r].item_start) * NW + wave;
+        if (i >= icount)
+            continue;
+
+        const EPI_CONST CohortDev *C = as_const(P.cohorts) + cohort;
+        const uint32_t M = C->M, Mp = C->Mp;
+        const uint32_t j0 = jg * JP;
+        const uint32_t k = kb * 64 + lane;
+
+        uint32_t acc0[JP][8], acc1[JP][8];
+        {
+            const uint32_t W = C->W[0];
+            const uint4 *S = C->sm[0];
+            count8<JP>(S + (size_t)i * W, S + (size_t)j0 * W, W, C->wm[0] + k, Mp, acc0);
+        }
+        {
+            const uint32_t W = C->W[1];
+            const uint4 *S = C->sm[1];
+            count8<JP>(S + (size_t)i * W, S + (size_t)j0 * W, W, C->wm[1] + k, Mp, acc1);
+        }
+
+        const uint32_t n0 = C->n[0], n1 = C->n[1];
+        const uint32_t tmax = P.table_len - 1;
+#pragma unroll
+        for (int jj = 0; jj < JP; ++jj) {
+            const uint32_t j = j0 + jj;
+            const bool valid = (i < j) && (j < k) && (k < M);
+            float s = INFINITY;
+            if (valid) {
+                uint32_t x0[27], x1[27];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const uint4 ij = ldc4(C->pair[c] + (size_t)i * Mp + j);
+                    const uint4 ik = C->pair[c][(size_t)i * Mp + k];
+                    const uint4 jk = C->pair[c][(size_t)j * Mp + k];
+                    const uint2 si = ldc2(C->single[c] + i);
+                    const uint2 sj = ldc2(C->single[c] + j);
+                    const uint2 sk = C->single[c][k];
+                    const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
+                    const uint32_t aik[4] = {ik.x, ik.y, ik.z, ik.w};
+                    const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
+                    const uint32_t asi[2] = {si.x, si.y};
+                    const uint32_t asj[2] = {sj.x, sj.y};
+                    const uint32_t ask[2] = {sk.x, sk.y};
+                    const uint32_t(&t8)[8] = c ? acc1[jj] : acc0[jj];
+                    derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n1 : n0, c ? x1 : x0);
+                }
+                // K2 in the reference order (GPU/v4.cpp:579-583, CPU/v2.cpp:516-519)
+                float sc = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 27; ++m) {
+                    // indices are <= N + 1 < table_len for exact counts; the
+                    // clamp only keeps a corrupted count inside the table
+                    const uint32_t ia = min(x0[m] + x1[m] + 1, tmax);
+                    const uint32_t ib = min(x0[m], tmax);
+                    const uint32_t id = min(x1[m], tmax);
+                    const float a = TLDS ? Tl[ia] : P.table[ia];
+                    const float b = TLDS ? Tl[ib] : P.table[ib];
+                    const float d = TLDS ? Tl[id] : P.table[id];
+                    sc += (a - b) - d;
+                    // bound the number of table loads in flight (register pressure)
+                    if (!TLDS && (m % 3) == 2)
+                        __builtin_amdgcn_sched_barrier(0);
+                }
+                s = fabsf(sc);
+                if (DUMP) {
+                    const uint64_t rk = rank3(M, i, j, k);
+                    P.dump_scores[rk] = s;
+                    if (P.dump_counts) {
+#pragma unroll
+                        for (int m = 0; m < 27; ++m) {
+                            P.dump_counts[rk * 54 + m] = x0[m];
+                            P.dump_counts[rk * 54 + 27 + m] = x1[m];
+                        }
+                    }
+                }
+            }
+            if (!DUMP)
+                wave_offer(valid, s, make_key(i, j, k), buf, keep, cnt, ts, tk, lane);
+        }
+    }
+}
+
+template <int NW>
+size_t lds_total(bool table_lds, uint32_t table_len)
+{
+    return scan_lds_fixed<NW>() + (table_lds ? (size_t)table_len * 4 : 0);
+}
+
+size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len)
+{
+    switch (cfg.waves) {
+    case 4: return lds_total<4>(table_lds, table_len);
+    case 8: return lds_total<8>(table_lds, table_len);
+    case 16: return lds_total<16>(table_lds, table_len);
+    default: return 0;
+    }
+}
+
+typedef void (*ScanFn)(const ScanParams);
+
+template <int JP, int NW>
+static ScanFn pick3(bool tl, bool dump)
+{
+    if (tl)
+        return dump ? scan_kernel<JP, NW, true, true> : scan_kernel<JP, NW, true, false>;
+    return dump ? scan_kernel<JP, NW, false, true> : scan_kernel<JP, NW, false, false>;
+}
+
+template <int JP>
+static ScanFn pick2(int nw, bool tl, bool dump)
+{
+    switch (nw) {
+    case 4: return pick3<JP, 4>(tl, dump);
+    case 8: return pick3<JP, 8>(tl, dump);
+    case 16: return pick3<JP, 16>(tl, dump);
+    default: return nullptr;
+    }
+}
+
+static ScanFn pick(const ScanConfig &cfg, bool tl, bool dump)
+{
+    if (cfg.words != 1)
+        return nullptr;
+    switch (cfg.snps_j) {
+    case 1: return pick2<1>(cfg.waves, tl, dump);
+    case 2: return pick2<2>(cfg.waves, tl, dump);
+    case 4: return pick2<4>(cfg.waves, tl, dump);
+    case 8: return pick2<8>(cfg.waves, tl, dump);
+    default: return nullptr;
+    }
+}
+
+bool scan_config_supported(const ScanConfig &cfg) { return pick(cfg, false, false) != nullptr; }
+
+hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes,
+                          int *blocks_per_cu)
+{
+    ScanFn f = pick(cfg, table_lds, dump);
+    if (!f)
+        return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)f,
+                                                        cfg.waves * 64, lds_bytes);
+}
+
+hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes)
+{
+    ScanFn f = pick(cfg, table_lds, dump);
+    if (!f)
+        return hipErrorInvalidValue;
+    return hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes);
+}
+
+hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump, const ScanParams &p,
+                       uint32_t grid, hipStream_t s)
+{
+    ScanFn f = pick(cfg, table_lds, dump);
+    if (!f)
+        return hipErrorInvalidValue;
+    const size_t lds = scan_lds_bytes(cfg, table_lds, p.table_len);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(cfg.waves * 64), lds, s, p);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// per-cohort merge of workgroup slots into the final top-kTop
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024)
+merge_kernel(const Cand *__restrict__ slots, const uint32_t *__restrict__ counters,
+             uint32_t slot_cap, Cand *__restrict__ merged)
+{
+    __shared__ Cand ent[1024];
+    const uint32_t cohort = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    uint32_t nslots = counters[1 + cohort];
+    if (nslots > slot_cap)
+        nslots = slot_cap;
+    const Cand *src = slots + (size_t)cohort * slot_cap * kTop;
+    Cand empty;
+    empty.s = INFINITY;
+    empty.pad = 0;
+    empty.key = kEmptyKey;
+    if (t < (uint32_t)kTop)
+        ent[t] = empty;
+    constexpr uint32_t per_pass = 1024 / kTop - 1;
+    for (uint32_t base = 0; base < nslots; base += per_pass) {
+        if (t >= (uint32_t)kTop) {
+            const uint32_t sl = base + (t - kTop) / kTop;
+            ent[t] = sl < nslots ? src[(size_t)sl * kTop + (t % kTop)] : empty;
+        }
+        __syncthreads();
+        const Cand me = ent[t];
+        const Cand thr = ent[kTop - 1];
+        uint32_t rank = kTop;
+        if (me.key != kEmptyKey && (t < (uint32_t)kTop || cand_less(me.s, me.key, thr.s, thr.key))) {
+            rank = 0;
+            for (uint32_t f = 0; f < 1024; ++f) {
+                const Cand o = ent[f];
+                rank += cand_less(o.s, o.key, me.s, me.key) ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+        if (t < (uint32_t)kTop)
+            ent[t] = empty;
+        __syncthreads();
+        if (rank < (uint32_t)kTop)
+            ent[rank] = me;
+        __syncthreads();
+    }
+    if (t < (uint32_t)kTop)
+        merged[(size_t)cohort * kTop + t] = ent[t];
+}
+
+hipError_t launch_merge(const Cand *slots, const uint32_t *counters, uint32_t slot_cap,
+                        uint32_t n_cohorts, Cand *merged, hipStream_t s)
+{
+    hipLaunchKernelGGL(merge_kernel, dim3(n_cohorts), dim3(1024), 0, s, slots, counters,
+                       slot_cap, merged);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// FP32 + FP64 rescoring of candidates with explicit 27-cell counting
+// (the reference's counting form, GPU/v4.cpp:321-577, on 64-bit words).
+// One wavefront per candidate.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ cands,
+               uint32_t per_cohort, const float *__restrict__ t32,
+               const double *__restrict__ t64, Rescored *__restrict__ out)
+{
+    const uint32_t e = blockIdx.x;
+    const uint32_t cohort = blockIdx.y;
+    const uint32_t lane = threadIdx.x;
+    const Cand cd = cands[(size_t)cohort * per_cohort + e];
+    Rescored *o = out + (size_t)cohort * per_cohort + e;
+    if (cd.key == kEmptyKey) {
+        if (lane == 0)
+            o->valid = 0;
+        return;
+    }
+    const uint32_t i = key_i(cd.key), j = key_j(cd.key), k = key_k(cd.key);
+    const CohortDev &C = cohorts[cohort];
+    uint32_t cnt[54];
+#pragma unroll
+    for (int m = 0; m < 54; ++m)
+        cnt[m] = 0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t W = C.W[c];
+        const uint32_t n = C.n[c];
+        const uint4 *S = C.sm[c];
+        for (uint32_t w = lane; w < W; w += 64) {
+            const uint4 a = S[(size_t)i * W + w];
+            const uint4 b = S[(size_t)j * W + w];
+            const uint4 d = S[(size_t)k * W + w];
+            const uint32_t rem = n - w * 64;
+            const uint64_t live = rem >= 64 ? ~0ull : ((1ull << rem) - 1ull);
+            uint64_t x[3], y[3], z[3];
+            x[0] = a.x | ((uint64_t)a.y << 32);
+            x[1] = a.z | ((uint64_t)a.w << 32);
+            x[2] = ~(x[0] | x[1]) & live;
+            y[0] = b.x | ((uint64_t)b.y << 32);
+            y[1] = b.z | ((uint64_t)b.w << 32);
+            y[2] = ~(y[0] | y[1]) & live;
+            z[0] = d.x | ((uint64_t)d.y << 32);
+            z[1] = d.z | ((uint64_t)d.w << 32);
+            z[2] = ~(z[0] | z[1]) & live;
+#pragma unroll
+            for (int gi = 0; gi < 3; ++gi)
+#pragma unroll
+                for (int gj = 0; gj < 3; ++gj)
+#pragma unroll
+                    for (int gk = 0; gk < 3; ++gk)
+                        cnt[27 * c + 9 * gi + 3 * gj + gk] += __popcll(x[gi] & y[gj] & z[gk]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 54; ++m) {
+        uint32_t v = cnt[m];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off);
+        cnt[m] = v;
+    }
+    if (lane == 0) {
+        float s32 = 0.0f;
+        double s64 = 0.0;
+#pragma unroll
+        for (int m = 0; m < 27; ++m) {
+            s32 += (t32[cnt[m] + cnt[27 + m] + 1] - t32[cnt[m]]) - t32[cnt[27 + m]];
+            s64 += (t64[cnt[m] + cnt[27 + m] + 1] - t64[cnt[m]]) - t64[cnt[27 + m]];
+        }
+        o->i = i;
+        o->j = j;
+        o->k = k;
+        o->valid = 1;
+        o->s32 = fabsf(s32);
+        o->s32_scan = cd.s;
+        o->s64 = fabs(s64);
+#pragma unroll
+        for (int m = 0; m < 54; ++m)
+            o->counts[m] = cnt[m];
+    }
+}
+
+hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands, uint32_t per_cohort,
+                          uint32_t n_cohorts, const float *t32, const double *t64,
+                          Rescored *out, hipStream_t s)
+{
+    if (per_cohort == 0 || n_cohorts == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(rescore_kernel, dim3(per_cohort, n_cohorts), dim3(64), 0, s, cohorts,
+                       cands, per_cohort, t32, t64, out);
+    return hipGetLastError();
+}
+
+}  // namespace epi

@@ -1,0 +1,311 @@
+"""ctypes binding of the episdet C ABI (include/episdet.h).
+
+The reference (hiperbio/cross-dpc-episdet) is a command-line program,
+``./v4 <num_samples> <num_snps> <block_size>`` (GPU/README.md:29-30); the
+``bin/v4`` program built next to this file is its drop-in.  This module is
+the same C ABI for Python callers (tests, the benchmark, notebooks): it
+loads ``lib/libepisdet.so`` and nothing else.  There is no fallback: when the
+library or a GPU is missing, calls raise ``EpiError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libepisdet.so")
+CLI_PATH = os.path.join(HERE, "bin", "v4")
+
+EPI_OK = 0
+EPI_EINVAL = -1
+EPI_ENOMEM = -2
+EPI_EDEVICE = -3
+EPI_EBUSY = -4
+EPI_ECANCELED = -5
+EPI_EAGAIN = -6
+EPI_ENOENT = -7
+EPI_EINTERNAL = -8
+EPI_TOPK = 64
+
+_u32 = ctypes.c_uint32
+
+
+class EpiCandidate(ctypes.Structure):
+    _fields_ = [
+        ("i", _u32), ("j", _u32), ("k", _u32),
+        ("score", ctypes.c_float),
+        ("score_fp64", ctypes.c_double),
+        ("counts", _u32 * 54),
+    ]
+
+
+class EpiResult(ctypes.Structure):
+    _fields_ = [
+        ("i", _u32), ("j", _u32), ("k", _u32),
+        ("score", ctypes.c_float),
+        ("score_fp64", ctypes.c_double),
+        ("triplets", ctypes.c_uint64),
+        ("n_samples", _u32), ("n_snps", _u32),
+        ("n_controls", _u32), ("n_cases", _u32),
+        ("scan_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("max_abs_err_fp32", ctypes.c_double),
+        ("fp64_order_differs", ctypes.c_int32),
+        ("n_top", _u32),
+        ("top", EpiCandidate * EPI_TOPK),
+    ]
+
+
+class EpiTile(ctypes.Structure):
+    _fields_ = [("variant", _u32), ("snps_j", _u32), ("waves", _u32), ("words", _u32)]
+
+
+# Every entry point of include/episdet.h: (name, restype, argtypes).
+_VP = ctypes.c_void_p
+_PROTOS = [
+    ("epi_version", ctypes.c_char_p, []),
+    ("epi_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("epi_open", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_VP)]),
+    ("epi_close", ctypes.c_int, [_VP]),
+    ("epi_device_name", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_size_t]),
+    ("epi_cohort_create", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(_VP)]),
+    ("epi_cohort_destroy", ctypes.c_int, [_VP]),
+    ("epi_cohort_bytes", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    ("epi_scan", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), ctypes.POINTER(EpiResult)]),
+    ("epi_scan_batch", ctypes.c_int, [_VP, ctypes.POINTER(_VP), _u32, ctypes.POINTER(EpiTile),
+                                      ctypes.POINTER(EpiResult)]),
+    ("epi_count", ctypes.c_int, [_VP, _VP, _VP, _u32, _VP, _VP, _VP]),
+    ("epi_scan_dump", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), _VP, _VP]),
+    ("epi_generate_reference", ctypes.c_int, [_u32, _u32, _VP, _VP]),
+    ("epi_generate_hwe", ctypes.c_int, [ctypes.c_uint64, _u32, _u32, ctypes.c_double,
+                                        ctypes.c_double, _u32, _VP, _VP]),
+    ("epi_generate_planted", ctypes.c_int, [ctypes.c_uint64, _u32, _u32, ctypes.c_double,
+                                            ctypes.c_double, _u32, _VP, _VP, _VP]),
+]
+
+
+class EpiError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load lib/libepisdet.so (built by `make` in this directory)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EpiError(EPI_EDEVICE, f"{LIB_PATH} is not built")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _PROTOS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def strerror(code: int) -> str:
+    try:
+        return lib().epi_strerror(code).decode()
+    except EpiError:
+        return f"error {code}"
+
+
+def _check(code: int, what: str) -> None:
+    if code != EPI_OK:
+        raise EpiError(code, what)
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _tile(tile) -> Optional[EpiTile]:
+    if tile is None:
+        return None
+    if isinstance(tile, EpiTile):
+        return tile
+    t = EpiTile()
+    for k, v in dict(tile).items():
+        setattr(t, k, int(v))
+    return t
+
+
+# ---------------------------------------------------------------- generators
+def generate_reference(n_samples: int, n_snps: int):
+    """The reference's srand(100) generator (GPU/v4.cpp:81-110)."""
+    g = np.empty((n_samples, n_snps), np.uint8)
+    p = np.empty(n_samples, np.uint8)
+    _check(lib().epi_generate_reference(n_samples, n_snps, _ptr(g), _ptr(p)), "generate_reference")
+    return g, p
+
+
+def generate_hwe(seed: int, n_samples: int, n_snps: int, n_cases: int,
+                 maf_lo: float = 0.05, maf_hi: float = 0.5):
+    g = np.empty((n_samples, n_snps), np.uint8)
+    p = np.empty(n_samples, np.uint8)
+    _check(lib().epi_generate_hwe(seed, n_samples, n_snps, maf_lo, maf_hi, n_cases, _ptr(g),
+                                  _ptr(p)), "generate_hwe")
+    return g, p
+
+
+def generate_planted(seed: int, n_samples: int, n_snps: int, n_cases: int,
+                     maf_lo: float = 0.05, maf_hi: float = 0.5):
+    g = np.empty((n_samples, n_snps), np.uint8)
+    p = np.empty(n_samples, np.uint8)
+    pl = np.zeros(3, np.uint32)
+    _check(lib().epi_generate_planted(seed, n_samples, n_snps, maf_lo, maf_hi, n_cases, _ptr(g),
+                                      _ptr(p), _ptr(pl)), "generate_planted")
+    return g, p, tuple(int(x) for x in pl)
+
+
+# ------------------------------------------------------------------ results
+@dataclass
+class Candidate:
+    i: int
+    j: int
+    k: int
+    score: float
+    score_fp64: float
+    counts: np.ndarray
+
+
+@dataclass
+class ScanResult:
+    i: int
+    j: int
+    k: int
+    score: float
+    score_fp64: float
+    triplets: int
+    n_samples: int
+    n_snps: int
+    n_controls: int
+    n_cases: int
+    scan_ms: float
+    kernel_ms: float
+    max_abs_err_fp32: float
+    fp64_order_differs: bool
+    top: List[Candidate] = field(default_factory=list)
+
+    @property
+    def best(self):
+        return (self.i, self.j, self.k)
+
+    @classmethod
+    def from_c(cls, r: EpiResult) -> "ScanResult":
+        top = []
+        for e in range(r.n_top):
+            c = r.top[e]
+            top.append(Candidate(c.i, c.j, c.k, float(c.score), float(c.score_fp64),
+                                 np.ctypeslib.as_array(c.counts).copy()))
+        return cls(r.i, r.j, r.k, float(r.score), float(r.score_fp64), int(r.triplets),
+                   r.n_samples, r.n_snps, r.n_controls, r.n_cases, float(r.scan_ms),
+                   float(r.kernel_ms), float(r.max_abs_err_fp32), bool(r.fp64_order_differs), top)
+
+
+# ------------------------------------------------------------------ objects
+class Context:
+    """An opened device (epi_open / epi_close)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().epi_open(device, ctypes.byref(h)), f"epi_open({device})")
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            lib().epi_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().epi_device_name(self.handle, buf, 256), "epi_device_name")
+        return buf.value.decode()
+
+    def cohort(self, geno: np.ndarray, pheno: np.ndarray) -> "Cohort":
+        return Cohort(self, geno, pheno)
+
+    def scan(self, cohort: "Cohort", tile=None) -> ScanResult:
+        r = EpiResult()
+        t = _tile(tile)
+        _check(lib().epi_scan(self.handle, cohort.handle, ctypes.byref(t) if t else None,
+                              ctypes.byref(r)), "epi_scan")
+        return ScanResult.from_c(r)
+
+    def scan_batch(self, cohorts: Sequence["Cohort"], tile=None) -> List[ScanResult]:
+        n = len(cohorts)
+        hs = (ctypes.c_void_p * n)(*[c.handle.value for c in cohorts])
+        rs = (EpiResult * n)()
+        t = _tile(tile)
+        _check(lib().epi_scan_batch(self.handle, hs, n, ctypes.byref(t) if t else None, rs),
+               "epi_scan_batch")
+        return [ScanResult.from_c(rs[b]) for b in range(n)]
+
+    def count(self, cohort: "Cohort", triplets):
+        tr = np.ascontiguousarray(np.asarray(triplets, dtype=np.uint32).reshape(-1, 3))
+        n = tr.shape[0]
+        counts = np.zeros((n, 54), np.uint32)
+        s32 = np.zeros(n, np.float32)
+        s64 = np.zeros(n, np.float64)
+        _check(lib().epi_count(self.handle, cohort.handle, _ptr(tr), n, _ptr(counts), _ptr(s32),
+                               _ptr(s64)), "epi_count")
+        return counts, s32, s64
+
+    def scan_dump(self, cohort: "Cohort", tile=None, with_counts: bool = False):
+        m = cohort.n_snps
+        nt = m * (m - 1) * (m - 2) // 6
+        scores = np.zeros(nt, np.float32)
+        counts = np.zeros((nt, 54), np.uint32) if with_counts else None
+        t = _tile(tile)
+        _check(lib().epi_scan_dump(self.handle, cohort.handle, ctypes.byref(t) if t else None,
+                                   _ptr(scores), _ptr(counts) if counts is not None else None),
+               "epi_scan_dump")
+        return scores, counts
+
+
+class Cohort:
+    """A bit-packed cohort resident in device memory (epi_cohort_create)."""
+
+    def __init__(self, ctx: Context, geno: np.ndarray, pheno: np.ndarray):
+        geno = np.ascontiguousarray(geno, dtype=np.uint8)
+        pheno = np.ascontiguousarray(pheno, dtype=np.uint8)
+        if geno.ndim != 2 or pheno.shape != (geno.shape[0],):
+            raise EpiError(EPI_EINVAL, "geno must be [n_samples][n_snps], pheno [n_samples]")
+        self.n_samples, self.n_snps = geno.shape
+        h = ctypes.c_void_p()
+        _check(lib().epi_cohort_create(ctx.handle, _ptr(geno), _ptr(pheno), self.n_samples,
+                                       self.n_snps, ctypes.byref(h)), "epi_cohort_create")
+        self.handle = h
+
+    @property
+    def device_bytes(self) -> int:
+        b = ctypes.c_uint64()
+        _check(lib().epi_cohort_bytes(self.handle, ctypes.byref(b)), "epi_cohort_bytes")
+        return int(b.value)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().epi_cohort_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,155 @@
+/*
+ * episdet.h -- C ABI of the MI355X-native exhaustive third-order epistasis
+ * scan with Bayesian K2 scoring.
+ *
+ * The reference (hiperbio/cross-dpc-episdet) has no library interface: its
+ * GPU path is one program, `./v4 <num_samples> <num_snps> <block_size>`
+ * (GPU/README.md:29-30, parsed at GPU/v4.cpp:210-212), whose kernel is a
+ * lambda inside main (GPU/v4.cpp:297-593).  Each entry point below names
+ * the piece of that program it replaces.  The `v4` command-line program
+ * shipped with this library is built on these entry points only.
+ *
+ * Conventions
+ *  - Every function returns 0 (EPI_OK) or a negative EPI_E* code; no C++
+ *    exception crosses this boundary.  epi_strerror() names the code.
+ *  - The caller owns every host buffer it passes; inputs may be freed as
+ *    soon as the call returns.  The library owns device memory until the
+ *    matching destroy/close call.
+ *  - A context is used by one host thread at a time, except the server
+ *    calls (epi_submit / epi_poll / epi_cancel), which are thread-safe.
+ *  - Genotypes are uint8 values 0/1/2, sample-major [n_samples][n_snps]
+ *    like the reference generator's matrix (GPU/v4.cpp:88-95); any value
+ *    other than 0 or 1 counts as genotype 2 (GPU/v4.cpp:156,170).
+ *    Phenotype value 1 marks a case; anything else is a control
+ *    (GPU/v4.cpp:141).
+ *  - Controls are class 0 and cases class 1, so counts[c] and
+ *    counts[27 + c] for cell c = 9*g_i + 3*g_j + g_k follow the
+ *    reference's frequency table ft[54] (GPU/v4.cpp:317-319).
+ */
+#ifndef EPISDET_H
+#define EPISDET_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPI_OK 0
+#define EPI_EINVAL (-1)     /* bad argument or shape */
+#define EPI_ENOMEM (-2)     /* host or device allocation failed */
+#define EPI_EDEVICE (-3)    /* HIP runtime or kernel error */
+#define EPI_EBUSY (-4)      /* server full / context busy */
+#define EPI_ECANCELED (-5)  /* request was cancelled */
+#define EPI_EAGAIN (-6)     /* result not ready yet (epi_poll) */
+#define EPI_ENOENT (-7)     /* unknown request id */
+#define EPI_EINTERNAL (-8)  /* internal consistency check failed */
+
+#define EPI_TOPK 64
+
+typedef struct epi_ctx epi_ctx;
+typedef struct epi_cohort epi_cohort;
+typedef struct epi_server epi_server;
+
+/* One scored triplet. */
+typedef struct {
+    uint32_t i, j, k;   /* SNP indices, i < j < k */
+    float score;        /* K2 score in FP32, summed in the reference order */
+    double score_fp64;  /* the same function evaluated in FP64 */
+    uint32_t counts[54];/* controls ft[0..26], cases ft[27..53] */
+} epi_candidate;
+
+/* Result of one scan.  Replaces the reference's final host argmin and
+ * print (GPU/v4.cpp:598-612); the reported triplet is the FP32 argmin,
+ * ties going to the lexicographically first (i,j,k). */
+typedef struct {
+    uint32_t i, j, k;
+    float score;
+    double score_fp64;
+    uint64_t triplets;        /* C(n_snps, 3) */
+    uint32_t n_samples, n_snps;
+    uint32_t n_controls, n_cases;
+    double scan_ms;           /* host wall time, launch to result on host */
+    double kernel_ms;         /* device time of the scan kernel */
+    double max_abs_err_fp32;  /* max |score - score_fp64| over top[] */
+    int32_t fp64_order_differs; /* 1 if FP64 ranks top[] differently */
+    uint32_t n_top;           /* entries of top[] that are filled */
+    epi_candidate top[EPI_TOPK]; /* best candidates by (score, i, j, k) */
+} epi_result;
+
+/* Tile shape of the scan kernel.  Zero fields are chosen by the library
+ * (autotuned table).  Replaces the reference's block_size argument
+ * (GPU/v4.cpp:212, nd_range at :277-278). */
+typedef struct {
+    uint32_t variant;   /* 0 auto, 1 VALU AND+popcount, 2 int8 MFMA */
+    uint32_t snps_j;    /* j SNPs per wavefront (1, 2, 4 or 8) */
+    uint32_t waves;     /* wavefronts per workgroup (1..16) */
+    uint32_t words;     /* 64-bit sample words per loop step (1 or 2) */
+} epi_tile;
+
+/* Library / device ---------------------------------------------------- */
+const char *epi_version(void);
+const char *epi_strerror(int code);
+
+/* Opens device `device` (replaces the SYCL queue creation,
+ * GPU/v4.cpp:243-251). */
+int epi_open(int device, epi_ctx **ctx);
+int epi_close(epi_ctx *ctx);
+/* Device name, as the reference prints it ("Selected device:",
+ * GPU/v4.cpp:251). */
+int epi_device_name(epi_ctx *ctx, char *buf, size_t len);
+
+/* Cohorts -------------------------------------------------------------- */
+/* Class split and bit-packing into resident device planes.  Replaces
+ * data_to_binary + binary_to_tiled + the USM uploads (GPU/v4.cpp:112-196,
+ * :220-230, :256-274).  n_snps >= 3, n_samples >= 1. */
+int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
+                      uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
+int epi_cohort_destroy(epi_cohort *cohort);
+/* Device bytes held by a cohort (its admission footprint). */
+int epi_cohort_bytes(const epi_cohort *cohort, uint64_t *bytes);
+
+/* Scans ---------------------------------------------------------------- */
+/* Exhaustive scan of all C(M,3) triplets.  Replaces the block-triple
+ * launch loop, the kernel and the host argmin (GPU/v4.cpp:283-608).
+ * tile may be NULL. */
+int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
+             epi_result *result);
+
+/* Many cohorts in one persistent launch over a shared (cohort, tile) work
+ * queue; results[b] is identical to epi_scan of cohorts[b]. */
+int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n,
+                   const epi_tile *tile, epi_result *results);
+
+/* 27x2 contingency counts of explicit triplets (3*n indices in, 54*n
+ * counts out), and their FP32/FP64 scores (either may be NULL). */
+int epi_count(epi_ctx *ctx, epi_cohort *cohort, const uint32_t *triplets,
+              uint32_t n, uint32_t *counts, float *scores, double *scores_fp64);
+
+/* Test hook: runs the scan kernel in dump mode and writes the FP32 score
+ * of every triplet at its lexicographic rank (scores has C(M,3) floats),
+ * and, if counts is not NULL, its 54 counts (54*C(M,3) uint32).  Only for
+ * n_snps <= 512. */
+int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
+                  float *scores, uint32_t *counts);
+
+/* Data generators ------------------------------------------------------- */
+/* The reference generator: srand(100), rand()%3, N/2 cases
+ * (GPU/v4.cpp:81-110).  geno is n_samples*n_snps, pheno n_samples. */
+int epi_generate_reference(uint32_t n_samples, uint32_t n_snps, uint8_t *geno,
+                           uint8_t *pheno);
+/* Seeded Hardy-Weinberg data, per-SNP MAF ~ U[maf_lo, maf_hi], exactly
+ * n_cases cases.  No reference counterpart. */
+int epi_generate_hwe(uint64_t seed, uint32_t n_samples, uint32_t n_snps,
+                     double maf_lo, double maf_hi, uint32_t n_cases,
+                     uint8_t *geno, uint8_t *pheno);
+/* Same with one planted three-way interaction (indices to planted[3]). */
+int epi_generate_planted(uint64_t seed, uint32_t n_samples, uint32_t n_snps,
+                         double maf_lo, double maf_hi, uint32_t n_cases,
+                         uint8_t *geno, uint8_t *pheno, uint32_t planted[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EPISDET_H */
