@@ -32,6 +32,7 @@ OPS_PER_TRIPLET_WORD = 54
 def load_binding():
     spec = importlib.util.spec_from_file_location("episdet", os.path.join(PKG, "episdet.py"))
     mod = importlib.util.module_from_spec(spec)
+    sys.modules["episdet"] = mod
     spec.loader.exec_module(mod)
     return mod
 

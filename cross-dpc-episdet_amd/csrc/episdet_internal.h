@@ -42,30 +42,51 @@ struct CohortDev {
     uint32_t pad[2];
 };
 
-// One (cohort, k-block, j-group) row of the triplet-tile queue.  Items of
-// the row are consecutive chunks of `waves` i values, i in [0, icount).
+// One (k-block, j-group) row of a cohort's triplet tiles.  Items of the
+// row are consecutive chunks of `waves` i values, i in [0, icount); item
+// numbers are local to the cohort's row table.
 struct Row {
-    uint32_t cohort;
+    uint32_t cohort;      // unused by the kernel (segments carry the cohort)
     uint32_t kb;          // k in [64 kb, 64 kb + 64)
     uint32_t jg;          // j in [snps_j * jg, snps_j * jg + snps_j)
     uint32_t icount;
-    uint32_t item_start;  // first queue item of this row
+    uint32_t item_start;  // first item of this row in the cohort's numbering
+    uint32_t pad;
+};
+
+// A contiguous range of one cohort's rows inside a launch.  Launch items
+// [item_base, item_base + n_items) map to the cohort's items starting at
+// item_first = rows[row_begin].item_start.
+struct Segment {
+    const Row *rows;      // the cohort's resident row table
+    uint32_t cohort;      // index into the launch's cohort descriptors
+    uint32_t row_begin, row_end;
+    uint32_t item_base;
+    uint32_t item_first;
+    uint32_t n_items;
+};
+
+// Work of one launch, read by the kernels from device memory so that a
+// captured graph can be replayed with new work.
+struct RoundHdr {
+    uint32_t n_segments;
+    uint32_t n_items;
+    uint32_t n_cohorts;
     uint32_t pad;
 };
 
 struct ScanParams {
-    const CohortDev *cohorts;
-    const Row *rows;
-    uint32_t n_rows;
-    uint32_t n_items;
-    uint32_t *counters;     // [0] item queue head, [1 + c] slot count of cohort c
-    const float *table;     // ln(n!) FP32, table_len entries
+    const RoundHdr *hdr;
+    const CohortDev *cohorts;  // [n_cohorts]
+    const Segment *segs;       // [n_segments]
+    uint32_t *counters;        // [0] item queue head, [1 + c] slots used by cohort c
+    const float *table;        // ln(n!) FP32, table_len entries
     uint32_t table_len;
-    uint32_t slot_cap;      // slots per cohort
-    Cand *slots;            // [cohort][slot_cap][kTop]
-    uint32_t *overflow;     // set to 1 if a slot could not be stored
-    float *dump_scores;     // dump mode: score per triplet rank (cohort 0)
-    uint32_t *dump_counts;  // dump mode: 54 counts per triplet rank
+    uint32_t slot_cap;         // slots per cohort
+    Cand *slots;               // [cohort][slot_cap][kTop]
+    uint32_t *overflow;        // set to 1 if a slot could not be stored
+    float *dump_scores;        // dump mode: score per triplet rank (cohort 0)
+    uint32_t *dump_counts;     // dump mode: 54 counts per triplet rank
 };
 
 // Rescoring output for one candidate.
@@ -100,11 +121,16 @@ size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len)
 hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes);
 hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump,
                        const ScanParams &p, uint32_t grid, hipStream_t s);
-hipError_t launch_merge(const Cand *slots, const uint32_t *counters,
-                        uint32_t slot_cap, uint32_t n_cohorts, Cand *merged,
+hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s);
+// Folds the workgroup slots of each cohort of the round into its running
+// top-kTop list lists[c] (read and written).
+hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
+                        uint32_t slot_cap, Cand *const *lists, uint32_t max_cohorts,
                         hipStream_t s);
+// Rescores per_cohort candidates of each cohort: cands given either as one
+// array [n_cohorts][per_cohort] or (lists != nullptr) as per-cohort lists.
 hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands,
-                          uint32_t per_cohort, uint32_t n_cohorts,
+                          Cand *const *lists, uint32_t per_cohort, uint32_t n_cohorts,
                           const float *t32, const double *t64, Rescored *out,
                           hipStream_t s);
 

@@ -186,6 +186,17 @@ hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &h, hipStre
 // ------------------------------------------------------------------------
 // scan kernel
 // ------------------------------------------------------------------------
+// acc += popcount(a) + popcount(b) as two chained v_bcnt_u32_b32 (popcount
+// and accumulate in one instruction each).  The empty asm pins the
+// intermediate sum in a VGPR; without it the compiler reassociates into
+// bcnt(a,0) + bcnt(b,0) + v_add3, one extra VALU op per cell-word.
+__device__ __forceinline__ void bcnt2(uint32_t &acc, uint32_t a, uint32_t b)
+{
+    uint32_t t = __popc(a) + acc;
+    asm("" : "+v"(t));
+    acc = __popc(b) + t;
+}
+
 // 8 cells (a, b, c in {0,1}) of JP triplets (i, j0 + jj, k) over one class.
 // x (SNP i) and y (SNP j) are wave-uniform: their loads are scalar and the
 // pair products are SALU work.  z (SNP k = lane) is a per-lane uint4.
@@ -199,7 +210,6 @@ __device__ __forceinline__ void count8(const uint4 *X, const uint4 *Y,
 #pragma unroll
         for (int e = 0; e < 8; ++e)
             acc[jj][e] = 0;
-#pragma unroll 2
     for (uint32_t w = 0; w < W; ++w) {
         const uint4 x = ldc4(X + w);
         const uint4 z = Z[(size_t)w * Mp];
@@ -210,14 +220,14 @@ __device__ __forceinline__ void count8(const uint4 *X, const uint4 *Y,
             const uint32_t p01l = x.x & y.z, p01h = x.y & y.w;  // g0(i) & g1(j)
             const uint32_t p10l = x.z & y.x, p10h = x.w & y.y;  // g1(i) & g0(j)
             const uint32_t p11l = x.z & y.z, p11h = x.w & y.w;  // g1(i) & g1(j)
-            acc[jj][0] += __popc(p00l & z.x) + __popc(p00h & z.y);
-            acc[jj][1] += __popc(p00l & z.z) + __popc(p00h & z.w);
-            acc[jj][2] += __popc(p01l & z.x) + __popc(p01h & z.y);
-            acc[jj][3] += __popc(p01l & z.z) + __popc(p01h & z.w);
-            acc[jj][4] += __popc(p10l & z.x) + __popc(p10h & z.y);
-            acc[jj][5] += __popc(p10l & z.z) + __popc(p10h & z.w);
-            acc[jj][6] += __popc(p11l & z.x) + __popc(p11h & z.y);
-            acc[jj][7] += __popc(p11l & z.z) + __popc(p11h & z.w);
+            bcnt2(acc[jj][0], p00l & z.x, p00h & z.y);
+            bcnt2(acc[jj][1], p00l & z.z, p00h & z.w);
+            bcnt2(acc[jj][2], p01l & z.x, p01h & z.y);
+            bcnt2(acc[jj][3], p01l & z.z, p01h & z.w);
+            bcnt2(acc[jj][4], p10l & z.x, p10h & z.y);
+            bcnt2(acc[jj][5], p10l & z.z, p10h & z.w);
+            bcnt2(acc[jj][6], p11l & z.x, p11h & z.y);
+            bcnt2(acc[jj][7], p11l & z.z, p11h & z.w);
         }
     }
 }
@@ -362,9 +372,13 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
     int cnt = 0;
     float ts = INFINITY;
     uint64_t tk = kEmptyKey;
-    uint32_t cur = 0xFFFFFFFFu;
-    uint32_t r = 0;
-    const EPI_CONST Row *rows = as_const(P.rows);
+    uint32_t cur = 0xFFFFFFFFu;  // cohort whose candidates the buffers hold
+    uint32_t sg = 0;             // current segment
+    uint32_t r = 0;              // current row of that segment
+    bool fresh = true;           // r must be reset to the segment's first row
+    const EPI_CONST Segment *segs = as_const(P.segs);
+    const uint32_t n_segs = as_const(P.hdr)->n_segments;
+    const uint32_t n_items = as_const(P.hdr)->n_items;
 
     for (;;) {
         if (tid == 0)
@@ -372,24 +386,39 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
         __syncthreads();
         const uint32_t item = rfl(L.misc[0]);
         __syncthreads();
+        // Items are pulled in increasing order, so segment and row only move
+        // forward: both are found by short forward walks.
         uint32_t cohort = 0xFFFFFFFFu;
-        if (item < P.n_items) {
-            while (r + 1 < P.n_rows && rows[r + 1].item_start <= item)
+        uint32_t u = 0;
+        const EPI_CONST Row *rows = nullptr;
+        if (item < n_items) {
+            while (sg + 1 < n_segs && segs[sg + 1].item_base <= item) {
+                ++sg;
+                fresh = true;
+            }
+            rows = (const EPI_CONST Row *)segs[sg].rows;
+            if (fresh) {
+                r = segs[sg].row_begin;
+                fresh = false;
+            }
+            u = item - segs[sg].item_base + segs[sg].item_first;
+            const uint32_t row_end = segs[sg].row_end;
+            while (r + 1 < row_end && rows[r + 1].item_start <= u)
                 ++r;
-            cohort = rows[r].cohort;
+            cohort = segs[sg].cohort;
         }
         if (cohort != cur) {
             if (!DUMP && cur != 0xFFFFFFFFu)
                 wg_flush<NW>(P, L, cur, cnt, ts, tk, wave, lane);
             cur = cohort;
         }
-        if (item >= P.n_items)
+        if (item >= n_items)
             break;
 
         const uint32_t kb = rows[r].kb;
         const uint32_t jg = rows[r].jg;
         const uint32_t icount = rows[r].icount;
-        const uint32_t i = (item - rows[r].item_start) * NW + wave;
+        const uint32_t i = (u - rows[r].item_start) * NW + wave;
         if (i >= icount)
             continue;
 
@@ -557,23 +586,51 @@ hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump, const S
 // ------------------------------------------------------------------------
 // per-cohort merge of workgroup slots into the final top-kTop
 // ------------------------------------------------------------------------
+__global__ void fill_empty_kernel(Cand *p, size_t n)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x) {
+        Cand e;
+        e.s = INFINITY;
+        e.pad = 0;
+        e.key = kEmptyKey;
+        p[t] = e;
+    }
+}
+
+hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024)
+        blocks = 1024;
+    hipLaunchKernelGGL(fill_empty_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// One workgroup per cohort: start from the cohort's running list, fold in
+// its slots 15 at a time (rank of each entry among 1024), write the list back.
 __global__ void __launch_bounds__(1024)
-merge_kernel(const Cand *__restrict__ slots, const uint32_t *__restrict__ counters,
-             uint32_t slot_cap, Cand *__restrict__ merged)
+merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
+             const uint32_t *__restrict__ counters, uint32_t slot_cap, Cand *const *lists)
 {
     __shared__ Cand ent[1024];
     const uint32_t cohort = blockIdx.x;
+    if (cohort >= hdr->n_cohorts)
+        return;
     const uint32_t t = threadIdx.x;
     uint32_t nslots = counters[1 + cohort];
     if (nslots > slot_cap)
         nslots = slot_cap;
     const Cand *src = slots + (size_t)cohort * slot_cap * kTop;
+    Cand *list = lists[cohort];
     Cand empty;
     empty.s = INFINITY;
     empty.pad = 0;
     empty.key = kEmptyKey;
     if (t < (uint32_t)kTop)
-        ent[t] = empty;
+        ent[t] = list[t];
     constexpr uint32_t per_pass = 1024 / kTop - 1;
     for (uint32_t base = 0; base < nslots; base += per_pass) {
         if (t >= (uint32_t)kTop) {
@@ -600,14 +657,17 @@ merge_kernel(const Cand *__restrict__ slots, const uint32_t *__restrict__ counte
         __syncthreads();
     }
     if (t < (uint32_t)kTop)
-        merged[(size_t)cohort * kTop + t] = ent[t];
+        list[t] = ent[t];
 }
 
-hipError_t launch_merge(const Cand *slots, const uint32_t *counters, uint32_t slot_cap,
-                        uint32_t n_cohorts, Cand *merged, hipStream_t s)
+hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
+                        uint32_t slot_cap, Cand *const *lists, uint32_t max_cohorts,
+                        hipStream_t s)
 {
-    hipLaunchKernelGGL(merge_kernel, dim3(n_cohorts), dim3(1024), 0, s, slots, counters,
-                       slot_cap, merged);
+    if (max_cohorts == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(merge_kernel, dim3(max_cohorts), dim3(1024), 0, s, hdr, slots, counters,
+                       slot_cap, lists);
     return hipGetLastError();
 }
 
@@ -618,13 +678,13 @@ hipError_t launch_merge(const Cand *slots, const uint32_t *counters, uint32_t sl
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
 rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ cands,
-               uint32_t per_cohort, const float *__restrict__ t32,
+               Cand *const *lists, uint32_t per_cohort, const float *__restrict__ t32,
                const double *__restrict__ t64, Rescored *__restrict__ out)
 {
     const uint32_t e = blockIdx.x;
     const uint32_t cohort = blockIdx.y;
     const uint32_t lane = threadIdx.x;
-    const Cand cd = cands[(size_t)cohort * per_cohort + e];
+    const Cand cd = lists ? lists[cohort][e] : cands[(size_t)cohort * per_cohort + e];
     Rescored *o = out + (size_t)cohort * per_cohort + e;
     if (cd.key == kEmptyKey) {
         if (lane == 0)
@@ -696,14 +756,14 @@ rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ c
     }
 }
 
-hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands, uint32_t per_cohort,
-                          uint32_t n_cohorts, const float *t32, const double *t64,
-                          Rescored *out, hipStream_t s)
+hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands, Cand *const *lists,
+                          uint32_t per_cohort, uint32_t n_cohorts, const float *t32,
+                          const double *t64, Rescored *out, hipStream_t s)
 {
     if (per_cohort == 0 || n_cohorts == 0)
         return hipSuccess;
     hipLaunchKernelGGL(rescore_kernel, dim3(per_cohort, n_cohorts), dim3(64), 0, s, cohorts,
-                       cands, per_cohort, t32, t64, out);
+                       cands, lists, per_cohort, t32, t64, out);
     return hipGetLastError();
 }
 

@@ -4,10 +4,25 @@
 
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/episdet.h"
 #include "episdet_internal.h"
+
+#define EPI_HIP(call)                   \
+    do {                                \
+        hipError_t e_ = (call);         \
+        if (e_ != hipSuccess)           \
+            return epi::hip_status(e_); \
+    } while (0)
+
+#define EPI_TRY(call)          \
+    do {                       \
+        int s_ = (call);       \
+        if (s_ != EPI_OK)      \
+            return s_;         \
+    } while (0)
 
 namespace epi {
 
@@ -23,14 +38,91 @@ struct DevBuf {
     void release();
 };
 
+// Pinned host staging buffer that only grows (async uploads of per-launch
+// tables).
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf();
+    int reserve(size_t want);
+};
+
+// One cohort's scan in progress: its triplet-tile rows (cohort-local item
+// numbering), how many rows have been launched, and its running top-kTop
+// candidate list on the device.
+struct Job {
+    epi_cohort *co = nullptr;
+    std::vector<Row> rows;  // host copy (ranges are chosen on the host)
+    DevBuf drows;           // resident copy read by the scan kernel
+    uint64_t items = 0;
+    uint32_t nw = 1;        // wavefronts per item (i values per item)
+    size_t next_row = 0;
+    DevBuf list;
+};
+
+// Tile table entry: applies to n_samples <= max_samples.
+struct TuneEntry {
+    uint32_t max_samples;
+    ScanConfig cfg;
+};
+
+// Launch geometry of the scan kernel for one configuration.
+struct LaunchPlan {
+    ScanConfig cfg;
+    bool table_lds = false;
+    bool dump = false;
+    size_t lds = 0;
+    uint32_t grid = 0;
+    uint32_t table_len = 0;
+};
+
 int hip_status(hipError_t e);
+int tune_load_file(epi_ctx *ctx, const std::string &path);
+void tune_load_default(epi_ctx *ctx);
+bool tune_lookup(const epi_ctx *ctx, uint32_t n_samples, ScanConfig &cfg);
+bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps);
+uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps);
+int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
+                         uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
 void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<double> &t64);
 int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);
 void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
                 std::vector<Row> &rows, uint64_t &items);
-int run_scans(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n, const epi_tile *tile,
-              epi_result *results, float *dump_scores, uint32_t *dump_counts);
+// Round tables in one device buffer: [hdr][segments][descs][lists], sized
+// for `cap` cohorts.
+struct RoundLayout {
+    size_t segs, descs, lists, total;
+};
+
+size_t align16(size_t x);
+RoundLayout round_layout(uint32_t cap);
+// Copies host memory to the device through the pinned staging buffer.
+int stage_upload(epi_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Host image of the round tables: one segment per job with rows
+// [ranges[b].first, .second) of jobs[b]; advances each job's next_row.
+int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint32_t n,
+                uint32_t cap, std::vector<unsigned char> &host, RoundLayout &L, uint64_t &items);
+int reserve_round(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, size_t bytes);
+// Queue reset, scan kernel and merge over the tables in ctx->round (fixed
+// device pointers only, so the sequence can be captured into a graph).
+int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, const RoundLayout &L,
+                          bool any_items, bool timed, float *dump_scores, uint32_t *dump_counts);
+int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool dump,
+                LaunchPlan &plan);
+int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg);
+int enqueue_pairs(epi_ctx *ctx, epi_cohort *co);
+// Uploads the round tables (rows [ranges[b].first, ranges[b].second) of
+// jobs[b]) and enqueues queue reset, scan kernel and merge.
+int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
+                  const std::pair<size_t, size_t> *ranges, uint32_t n, bool timed,
+                  float *dump_scores, uint32_t *dump_counts);
+// Rescoring of the running lists of jobs, copy back, epi_result assembly.
+int finish_jobs(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
+                double scan_ms, double kernel_ms);
 
 }  // namespace epi
 
@@ -45,8 +137,15 @@ struct epi_ctx {
     // ln(n!) tables (FP32 reference semantics, FP64 twin), table_len entries
     uint32_t table_len = 0;
     epi::DevBuf t32, t64;
-    // per-scan scratch, grown on demand
-    epi::DevBuf descs, rows, counters, slots, merged, rescored;
+    std::vector<epi::TuneEntry> tune;  // autotuned tile table
+    // per-launch scratch, grown on demand
+    epi::DevBuf round, fin, counters, slots, rescored;
+    epi::HostBuf stage;
+    hipEvent_t stage_ev = nullptr;  // marks the last copy out of `stage`
+    bool stage_pending = false;
+    // captured batch graph, reused while the batch signature is unchanged
+    hipGraphExec_t batch_graph = nullptr;
+    std::vector<uint64_t> batch_sig;
     ~epi_ctx();
 };
 

@@ -1,0 +1,601 @@
+// scan.cpp -- scans behind the C ABI: triplet-tile rows, launch planning,
+// rounds (one persistent scan launch over the rows of several cohorts),
+// graph-captured batches, rescoring and result assembly.
+//
+// Replaces the reference's host launch loop and argmin (GPU/v4.cpp:283-608):
+// there, one kernel per SNP-block triple with a queue.wait() and a parameter
+// copy before each launch, then a host scan over bs^3 slots.  Here, one
+// launch pulls every tile from a device queue and the result is a merged
+// top-64 list per cohort.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/episdet.h"
+#include "episdet_internal.h"
+#include "episdet_lib.h"
+
+namespace epi {
+
+static uint64_t choose3(uint64_t m) { return m < 3 ? 0 : m * (m - 1) * (m - 2) / 6; }
+
+static double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+HostBuf::~HostBuf()
+{
+    if (ptr)
+        (void)hipHostFree(ptr);
+}
+
+int HostBuf::reserve(size_t want)
+{
+    if (want <= bytes && ptr)
+        return EPI_OK;
+    if (ptr)
+        (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e);
+    }
+    bytes = want;
+    return EPI_OK;
+}
+
+// Triplet-tile rows of one cohort: k-block kb (64 k), j-group jg (snps_j
+// j), i in [0, icount).  Only rows holding at least one i < j < k < M.
+void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
+                std::vector<Row> &rows, uint64_t &items)
+{
+    for (uint32_t kb = 0; kb * 64 < Mp; ++kb) {
+        const uint32_t k_lo = kb * 64;
+        if (k_lo >= M)
+            break;
+        const uint32_t k_hi = std::min(kb * 64 + 63, M - 1);
+        for (uint32_t jg = 0;; ++jg) {
+            const uint32_t j_lo = jg * (uint32_t)jp;
+            if (j_lo >= k_hi || j_lo > M - 2)
+                break;
+            const uint32_t j_max = std::min(std::min(j_lo + (uint32_t)jp - 1, k_hi - 1), M - 2);
+            const uint32_t icount = j_max;
+            if (icount == 0)
+                continue;
+            Row r;
+            r.cohort = cohort;
+            r.kb = kb;
+            r.jg = jg;
+            r.icount = icount;
+            r.item_start = (uint32_t)items;
+            r.pad = 0;
+            rows.push_back(r);
+            items += (icount + (uint32_t)nw - 1) / (uint32_t)nw;
+        }
+    }
+}
+
+// Tile shape: explicit fields of `tile` win; then the autotuned table of
+// the context; otherwise a default that follows the table placement: with
+// the ln(n!) table in LDS, 4 j-SNPs per wavefront fit in 128 VGPRs; a large
+// table in LDS allows one workgroup per CU, so it gets 16 wavefronts; with
+// the table in global memory, 2 j-SNPs per wavefront.
+ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
+{
+    ScanConfig cfg;
+    cfg.words = 1;
+    cfg.snps_j = 4;
+    cfg.waves = 8;
+    if (!tune_lookup(ctx, n_samples, cfg)) {
+        ScanConfig probe = cfg;
+        if (scan_lds_bytes(probe, true, n_samples + 2) > 64 * 1024) {
+            probe.waves = 16;
+            if (scan_lds_bytes(probe, true, n_samples + 2) <= ctx->lds_per_block)
+                cfg.waves = 16;
+            else
+                cfg.snps_j = 2;
+        }
+    }
+    if (tile) {
+        if (tile->snps_j)
+            cfg.snps_j = (int)tile->snps_j;
+        if (tile->waves)
+            cfg.waves = (int)tile->waves;
+        if (tile->words)
+            cfg.words = (int)tile->words;
+    }
+    return cfg;
+}
+
+int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool dump,
+                LaunchPlan &plan)
+{
+    if (!scan_config_supported(cfg))
+        return EPI_EINVAL;
+    plan.cfg = cfg;
+    plan.dump = dump;
+    plan.table_len = table_len;
+    plan.table_lds = scan_lds_bytes(cfg, true, table_len) <= ctx->lds_per_block;
+    plan.lds = scan_lds_bytes(cfg, plan.table_lds, table_len);
+    if (plan.lds > 64 * 1024)
+        EPI_HIP(scan_allow_lds(cfg, plan.table_lds, dump, plan.lds));
+    int per_cu = 0;
+    EPI_HIP(scan_occupancy(cfg, plan.table_lds, dump, plan.lds, &per_cu));
+    if (per_cu <= 0)
+        return EPI_EDEVICE;
+    plan.grid = (uint32_t)per_cu * (uint32_t)ctx->cu_count;
+    return EPI_OK;
+}
+
+int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg)
+{
+    job.co = co;
+    job.rows.clear();
+    job.items = 0;
+    job.next_row = 0;
+    build_rows(0, co->M, co->Mp, cfg.snps_j, cfg.waves, job.rows, job.items);
+    if (job.items >= 0x7FFFFFFFull)
+        return EPI_EINVAL;
+    job.nw = (uint32_t)cfg.waves;
+    EPI_TRY(job.list.reserve(sizeof(Cand) * kTop));
+    EPI_HIP(launch_fill_empty((Cand *)job.list.ptr, kTop, ctx->stream));
+    EPI_TRY(job.drows.reserve(sizeof(Row) * std::max<size_t>(job.rows.size(), 1)));
+    EPI_TRY(stage_upload(ctx, job.drows.ptr, job.rows.data(), sizeof(Row) * job.rows.size()));
+    return EPI_OK;
+}
+
+int enqueue_pairs(epi_ctx *ctx, epi_cohort *co)
+{
+    const CohortDev h = co->host_desc();
+    EPI_HIP(launch_pairs((const CohortDev *)co->desc.ptr, h, ctx->stream));
+    return EPI_OK;
+}
+
+// Copies `bytes` from host memory to `dst` through the pinned staging
+// buffer.  The staging buffer is reused, so the previous upload must have
+// left it first.
+int stage_upload(epi_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0)
+        return EPI_OK;
+    if (ctx->stage_pending) {
+        EPI_HIP(hipEventSynchronize(ctx->stage_ev));
+        ctx->stage_pending = false;
+    }
+    EPI_TRY(ctx->stage.reserve(bytes));
+    std::memcpy(ctx->stage.ptr, src, bytes);
+    EPI_HIP(hipMemcpyAsync(dst, ctx->stage.ptr, bytes, hipMemcpyHostToDevice, ctx->stream));
+    EPI_HIP(hipEventRecord(ctx->stage_ev, ctx->stream));
+    ctx->stage_pending = true;
+    return EPI_OK;
+}
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+RoundLayout round_layout(uint32_t cap)
+{
+    RoundLayout L;
+    L.segs = align16(sizeof(RoundHdr));
+    L.descs = L.segs + align16(sizeof(Segment) * cap);
+    L.lists = L.descs + align16(sizeof(CohortDev) * cap);
+    L.total = L.lists + align16(sizeof(Cand *) * cap);
+    return L;
+}
+
+// Host image of the round tables: one segment per job with rows
+// [ranges[b].first, .second) of jobs[b]; advances each job's next_row.
+int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint32_t n,
+                       uint32_t cap, std::vector<unsigned char> &host, RoundLayout &L,
+                       uint64_t &items)
+{
+    L = round_layout(cap);
+    host.assign(L.total, 0);
+    RoundHdr *hdr = (RoundHdr *)host.data();
+    Segment *segs = (Segment *)(host.data() + L.segs);
+    CohortDev *descs = (CohortDev *)(host.data() + L.descs);
+    Cand **lists = (Cand **)(host.data() + L.lists);
+    items = 0;
+    uint32_t ns = 0;
+    for (uint32_t b = 0; b < n; ++b) {
+        const Job &J = *jobs[b];
+        descs[b] = J.co->host_desc();
+        lists[b] = (Cand *)J.list.ptr;
+        const size_t r0 = ranges[b].first, r1 = ranges[b].second;
+        if (r1 <= r0)
+            continue;
+        const Row &last = J.rows[r1 - 1];
+        const uint32_t first = J.rows[r0].item_start;
+        const uint32_t end = last.item_start + (last.icount + J.nw - 1) / J.nw;
+        Segment sg;
+        sg.rows = (const Row *)J.drows.ptr;
+        sg.cohort = b;
+        sg.row_begin = (uint32_t)r0;
+        sg.row_end = (uint32_t)r1;
+        sg.item_base = (uint32_t)items;
+        sg.item_first = first;
+        sg.n_items = end - first;
+        segs[ns++] = sg;
+        items += sg.n_items;
+    }
+    if (items >= 0x7FFFFFFFull)
+        return EPI_EINVAL;
+    hdr->n_segments = ns;
+    hdr->n_items = (uint32_t)items;
+    hdr->n_cohorts = n;
+    for (uint32_t b = 0; b < n; ++b)
+        jobs[b]->next_row = ranges[b].second;
+    return EPI_OK;
+}
+
+int reserve_round(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, size_t bytes)
+{
+    EPI_TRY(ctx->round.reserve(bytes));
+    EPI_TRY(ctx->counters.reserve(sizeof(uint32_t) * (2 + (size_t)n)));
+    EPI_TRY(ctx->slots.reserve(sizeof(Cand) * (size_t)n * plan.grid * kTop));
+    return EPI_OK;
+}
+
+// Queue reset, scan kernel and merge over the tables in ctx->round.  Only
+// device pointers that stay fixed for a given buffer set are passed, so the
+// sequence can be captured into a graph.
+int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, const RoundLayout &L,
+                          bool any_items, bool timed, float *dump_scores, uint32_t *dump_counts)
+{
+    unsigned char *base = (unsigned char *)ctx->round.ptr;
+    uint32_t *counters = (uint32_t *)ctx->counters.ptr;
+    EPI_HIP(launch_fill_u32(counters, 0u, 2 + (size_t)n, ctx->stream));
+    ScanParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.hdr = (const RoundHdr *)base;
+    P.cohorts = (const CohortDev *)(base + L.descs);
+    P.segs = (const Segment *)(base + L.segs);
+    P.counters = counters;
+    P.table = (const float *)ctx->t32.ptr;
+    P.table_len = plan.table_len;
+    P.slot_cap = plan.grid;
+    P.slots = (Cand *)ctx->slots.ptr;
+    P.overflow = counters + 1 + n;
+    P.dump_scores = dump_scores;
+    P.dump_counts = dump_counts;
+    if (timed)
+        EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (any_items)
+        EPI_HIP(launch_scan(plan.cfg, plan.table_lds, plan.dump, P, plan.grid, ctx->stream));
+    if (timed)
+        EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!plan.dump)
+        EPI_HIP(launch_merge(P.hdr, P.slots, counters, plan.grid, (Cand *const *)(base + L.lists),
+                             n, ctx->stream));
+    return EPI_OK;
+}
+
+int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
+                  const std::pair<size_t, size_t> *ranges, uint32_t n, bool timed,
+                  float *dump_scores, uint32_t *dump_counts)
+{
+    std::vector<unsigned char> host;
+    RoundLayout L;
+    uint64_t items = 0;
+    EPI_TRY(build_round(jobs, ranges, n, n, host, L, items));
+    EPI_TRY(reserve_round(ctx, plan, n, L.total));
+    EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
+    return enqueue_round_kernels(ctx, plan, n, L, items > 0, timed, dump_scores, dump_counts);
+}
+
+// Rescoring tables [descs][lists] of the jobs, in ctx->fin.
+static int upload_rescore(epi_ctx *ctx, Job *const *jobs, uint32_t n)
+{
+    const size_t ld = align16(sizeof(CohortDev) * n);
+    const size_t total = ld + sizeof(Cand *) * n;
+    std::vector<unsigned char> host(total, 0);
+    for (uint32_t b = 0; b < n; ++b) {
+        ((CohortDev *)host.data())[b] = jobs[b]->co->host_desc();
+        ((Cand **)(host.data() + ld))[b] = (Cand *)jobs[b]->list.ptr;
+    }
+    EPI_TRY(ctx->fin.reserve(total));
+    EPI_TRY(ctx->rescored.reserve(sizeof(Rescored) * (size_t)n * kTop));
+    return stage_upload(ctx, ctx->fin.ptr, host.data(), total);
+}
+
+static int enqueue_rescore_kernel(epi_ctx *ctx, uint32_t n)
+{
+    const size_t ld = align16(sizeof(CohortDev) * n);
+    const unsigned char *base = (const unsigned char *)ctx->fin.ptr;
+    EPI_HIP(launch_rescore((const CohortDev *)base, nullptr, (Cand *const *)(base + ld), kTop, n,
+                           (const float *)ctx->t32.ptr, (const double *)ctx->t64.ptr,
+                           (Rescored *)ctx->rescored.ptr, ctx->stream));
+    return EPI_OK;
+}
+
+static int enqueue_rescore(epi_ctx *ctx, Job *const *jobs, uint32_t n)
+{
+    EPI_TRY(upload_rescore(ctx, jobs, n));
+    return enqueue_rescore_kernel(ctx, n);
+}
+
+static int assemble(const Rescored *rs, const epi_cohort *co, epi_result &R, double scan_ms,
+                    double kernel_ms)
+{
+    std::memset(&R, 0, sizeof(R));
+    R.triplets = choose3(co->M);
+    R.n_samples = co->N;
+    R.n_snps = co->M;
+    R.n_controls = co->n[0];
+    R.n_cases = co->n[1];
+    R.scan_ms = scan_ms;
+    R.kernel_ms = kernel_ms;
+    uint32_t nt = 0;
+    double maxerr = 0.0;
+    for (int e = 0; e < kTop; ++e) {
+        if (!rs[e].valid)
+            break;
+        // the scan's FP32 score must equal the explicit 27-cell recount
+        if (std::memcmp(&rs[e].s32, &rs[e].s32_scan, sizeof(float)) != 0)
+            return EPI_EINTERNAL;
+        epi_candidate &c = R.top[nt++];
+        c.i = rs[e].i;
+        c.j = rs[e].j;
+        c.k = rs[e].k;
+        c.score = rs[e].s32;
+        c.score_fp64 = rs[e].s64;
+        std::memcpy(c.counts, rs[e].counts, sizeof(c.counts));
+        maxerr = std::max(maxerr, std::fabs((double)c.score - c.score_fp64));
+    }
+    R.n_top = nt;
+    R.max_abs_err_fp32 = maxerr;
+    if (nt > 0) {
+        R.i = R.top[0].i;
+        R.j = R.top[0].j;
+        R.k = R.top[0].k;
+        R.score = R.top[0].score;
+        R.score_fp64 = R.top[0].score_fp64;
+        // near-tie flag: does FP64 order the FP32-ranked candidates differently?
+        for (uint32_t e = 0; e + 1 < nt; ++e)
+            if (R.top[e + 1].score_fp64 < R.top[e].score_fp64)
+                R.fp64_order_differs = 1;
+    }
+    return EPI_OK;
+}
+
+static int copy_back_and_assemble(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
+                                  std::chrono::steady_clock::time_point t0, bool timed)
+{
+    std::vector<Rescored> host((size_t)n * kTop);
+    uint32_t overflow = 0;
+    EPI_HIP(hipMemcpyAsync(host.data(), ctx->rescored.ptr, sizeof(Rescored) * host.size(),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    EPI_HIP(hipMemcpyAsync(&overflow, (uint32_t *)ctx->counters.ptr + 1 + n, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    EPI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_pending = false;
+    const double wall = ms_since(t0);
+    float kms = 0.0f;
+    if (timed)
+        EPI_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    if (overflow != 0)
+        return EPI_EINTERNAL;  // a workgroup slot was lost
+    for (uint32_t b = 0; b < n; ++b)
+        EPI_TRY(assemble(host.data() + (size_t)b * kTop, jobs[b]->co, results[b], wall, kms));
+    return EPI_OK;
+}
+
+int finish_jobs(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
+                double scan_ms, double kernel_ms)
+{
+    EPI_TRY(enqueue_rescore(ctx, jobs, n));
+    std::vector<Rescored> host((size_t)n * kTop);
+    EPI_HIP(hipMemcpyAsync(host.data(), ctx->rescored.ptr, sizeof(Rescored) * host.size(),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    EPI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_pending = false;
+    for (uint32_t b = 0; b < n; ++b)
+        EPI_TRY(assemble(host.data() + (size_t)b * kTop, jobs[b]->co, results[b], scan_ms,
+                         kernel_ms));
+    return EPI_OK;
+}
+
+static int check_cohorts(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n)
+{
+    if (!ctx || !cohorts || n == 0)
+        return EPI_EINVAL;
+    for (uint32_t b = 0; b < n; ++b)
+        if (!cohorts[b] || cohorts[b]->ctx != ctx)
+            return EPI_EINVAL;
+    return EPI_OK;
+}
+
+}  // namespace epi
+
+using namespace epi;
+
+extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
+                        epi_result *result)
+{
+    if (!result)
+        return EPI_EINVAL;
+    EPI_TRY(check_cohorts(ctx, &cohort, 1));
+    if (tile && tile->variant > 1)
+        return EPI_EINVAL;  // only the VALU popcount path is built
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EPI_HIP(hipSetDevice(ctx->device));
+    EPI_TRY(ctx_ensure_tables(ctx, cohort->N));
+    const ScanConfig cfg = resolve_config(ctx, tile, cohort->N);
+    LaunchPlan plan;
+    EPI_TRY(plan_launch(ctx, cfg, cohort->N + 2, false, plan));
+    const auto t0 = std::chrono::steady_clock::now();
+    Job job;
+    EPI_TRY(job_init(ctx, job, cohort, cfg));
+    EPI_TRY(enqueue_pairs(ctx, cohort));
+    Job *jp[1] = {&job};
+    const std::pair<size_t, size_t> all(0, job.rows.size());
+    EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, true, nullptr, nullptr));
+    EPI_TRY(enqueue_rescore(ctx, jp, 1));
+    return copy_back_and_assemble(ctx, jp, 1, result, t0, true);
+}
+
+// All cohorts in one persistent launch over one queue.  The kernel sequence
+// (queue reset, scan, merge, rescoring) is captured into a HIP graph once
+// and replayed while the batch keeps the same shape and buffers; the round
+// tables it reads are uploaded before each replay.
+extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n,
+                              const epi_tile *tile, epi_result *results)
+{
+    if (!results)
+        return EPI_EINVAL;
+    EPI_TRY(check_cohorts(ctx, cohorts, n));
+    if (tile && tile->variant > 1)
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EPI_HIP(hipSetDevice(ctx->device));
+    uint32_t n_max = 0;
+    for (uint32_t b = 0; b < n; ++b)
+        n_max = std::max(n_max, cohorts[b]->N);
+    EPI_TRY(ctx_ensure_tables(ctx, n_max));
+    const ScanConfig cfg = resolve_config(ctx, tile, n_max);
+    LaunchPlan plan;
+    EPI_TRY(plan_launch(ctx, cfg, n_max + 2, false, plan));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<Job> jobs(n);
+    std::vector<Job *> jp(n);
+    std::vector<std::pair<size_t, size_t>> ranges(n);
+    for (uint32_t b = 0; b < n; ++b) {
+        EPI_TRY(job_init(ctx, jobs[b], cohorts[b], cfg));
+        EPI_TRY(enqueue_pairs(ctx, cohorts[b]));
+        jp[b] = &jobs[b];
+        ranges[b] = std::make_pair((size_t)0, jobs[b].rows.size());
+    }
+    std::vector<unsigned char> host;
+    RoundLayout L;
+    uint64_t items = 0;
+    EPI_TRY(build_round(jp.data(), ranges.data(), n, n, host, L, items));
+    EPI_TRY(reserve_round(ctx, plan, n, L.total));
+    EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
+    EPI_TRY(upload_rescore(ctx, jp.data(), n));
+
+    std::vector<uint64_t> sig = {(uint64_t)n, (uint64_t)(items > 0), (uint64_t)plan.cfg.snps_j,
+                                 (uint64_t)plan.cfg.waves, (uint64_t)plan.grid, plan.table_len,
+                                 (uint64_t)plan.table_lds, L.total,
+                                 (uint64_t)(uintptr_t)ctx->round.ptr,
+                                 (uint64_t)(uintptr_t)ctx->counters.ptr,
+                                 (uint64_t)(uintptr_t)ctx->slots.ptr,
+                                 (uint64_t)(uintptr_t)ctx->fin.ptr,
+                                 (uint64_t)(uintptr_t)ctx->rescored.ptr,
+                                 (uint64_t)(uintptr_t)ctx->t32.ptr,
+                                 (uint64_t)(uintptr_t)ctx->t64.ptr};
+    if (!ctx->batch_graph || sig != ctx->batch_sig) {
+        if (ctx->batch_graph) {
+            (void)hipGraphExecDestroy(ctx->batch_graph);
+            ctx->batch_graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        EPI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_round_kernels(ctx, plan, n, L, items > 0, false, nullptr, nullptr);
+        if (st == EPI_OK)
+            st = enqueue_rescore_kernel(ctx, n);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (st != EPI_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return st;
+        }
+        EPI_HIP(ce);
+        hipError_t ie = hipGraphInstantiate(&ctx->batch_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        EPI_HIP(ie);
+        ctx->batch_sig = sig;
+    }
+    EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    EPI_HIP(hipGraphLaunch(ctx->batch_graph, ctx->stream));
+    EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    return copy_back_and_assemble(ctx, jp.data(), n, results, t0, true);
+}
+
+extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
+                             float *scores, uint32_t *counts)
+{
+    if (!scores)
+        return EPI_EINVAL;
+    EPI_TRY(check_cohorts(ctx, &cohort, 1));
+    if (cohort->M > 512 || (tile && tile->variant > 1))
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EPI_HIP(hipSetDevice(ctx->device));
+    EPI_TRY(ctx_ensure_tables(ctx, cohort->N));
+    const ScanConfig cfg = resolve_config(ctx, tile, cohort->N);
+    LaunchPlan plan;
+    EPI_TRY(plan_launch(ctx, cfg, cohort->N + 2, true, plan));
+    const uint64_t ntrip = choose3(cohort->M);
+    DevBuf ds, dc;
+    EPI_TRY(ds.reserve(std::max<uint64_t>(ntrip, 1) * sizeof(float)));
+    if (counts)
+        EPI_TRY(dc.reserve(std::max<uint64_t>(ntrip, 1) * 54 * sizeof(uint32_t)));
+    EPI_HIP(launch_fill_u32((uint32_t *)ds.ptr, 0x7F800000u, ntrip, ctx->stream));  // +inf
+    if (counts)
+        EPI_HIP(launch_fill_u32((uint32_t *)dc.ptr, 0xFFFFFFFFu, ntrip * 54, ctx->stream));
+    Job job;
+    EPI_TRY(job_init(ctx, job, cohort, cfg));
+    EPI_TRY(enqueue_pairs(ctx, cohort));
+    Job *jp[1] = {&job};
+    const std::pair<size_t, size_t> all(0, job.rows.size());
+    EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, false, (float *)ds.ptr, (uint32_t *)dc.ptr));
+    EPI_HIP(hipMemcpyAsync(scores, ds.ptr, ntrip * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    if (counts)
+        EPI_HIP(hipMemcpyAsync(counts, dc.ptr, ntrip * 54 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    EPI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_pending = false;
+    return EPI_OK;
+}
+
+extern "C" int epi_count(epi_ctx *ctx, epi_cohort *co, const uint32_t *triplets, uint32_t n,
+                         uint32_t *counts, float *scores, double *scores_fp64)
+{
+    if (!ctx || !co || co->ctx != ctx || (!triplets && n))
+        return EPI_EINVAL;
+    if (n == 0)
+        return EPI_OK;
+    for (uint32_t t = 0; t < n; ++t) {
+        const uint32_t i = triplets[3 * t], j = triplets[3 * t + 1], k = triplets[3 * t + 2];
+        if (!(i < j && j < k && k < co->M))
+            return EPI_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EPI_HIP(hipSetDevice(ctx->device));
+    EPI_TRY(ctx_ensure_tables(ctx, co->N));
+    std::vector<Cand> cands(n);
+    for (uint32_t t = 0; t < n; ++t) {
+        cands[t].s = 0.0f;
+        cands[t].pad = 0;
+        cands[t].key = make_key(triplets[3 * t], triplets[3 * t + 1], triplets[3 * t + 2]);
+    }
+    DevBuf dc, dr;
+    EPI_TRY(dc.reserve(sizeof(Cand) * n));
+    EPI_TRY(dr.reserve(sizeof(Rescored) * n));
+    EPI_HIP(hipMemcpyAsync(dc.ptr, cands.data(), sizeof(Cand) * n, hipMemcpyHostToDevice,
+                           ctx->stream));
+    EPI_HIP(launch_rescore((const CohortDev *)co->desc.ptr, (const Cand *)dc.ptr, nullptr, n, 1,
+                           (const float *)ctx->t32.ptr, (const double *)ctx->t64.ptr,
+                           (Rescored *)dr.ptr, ctx->stream));
+    std::vector<Rescored> host(n);
+    EPI_HIP(hipMemcpyAsync(host.data(), dr.ptr, sizeof(Rescored) * n, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    EPI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stage_pending = false;
+    for (uint32_t t = 0; t < n; ++t) {
+        if (counts)
+            std::memcpy(counts + 54 * (size_t)t, host[t].counts, sizeof(host[t].counts));
+        if (scores)
+            scores[t] = host[t].s32;
+        if (scores_fp64)
+            scores_fp64[t] = host[t].s64;
+    }
+    return EPI_OK;
+}

@@ -1,0 +1,468 @@
+// server.cpp -- continuous-batching request loop behind the C ABI.
+//
+// No reference counterpart: the reference scans one generated dataset per
+// process (GPU/v4.cpp:198-624).  Here a worker thread owns the context's
+// stream and runs scheduling rounds:
+//   1. admission: queued requests are packed into HBM (cohort planes, pair
+//      margins, resident row table) in FIFO order while the device
+//      footprint stays within the budget and fewer than max_active run;
+//   2. one round: every active request gets a fair share of a work budget
+//      (sized from the measured throughput to take about round_ms of device
+//      time) as a contiguous range of its remaining tile rows; the ranges
+//      of all requests become the segments of one persistent scan launch,
+//      replayed from a captured HIP graph (queue reset, scan, merge);
+//   3. requests whose rows are exhausted are rescored and completed; their
+//      device memory is released before the next admission.
+// Every triplet of a request is scored exactly once whatever the round
+// split, and each round folds the new candidates into the request's exact
+// running top-64, so a result is identical to epi_scan of the same data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/episdet.h"
+#include "episdet_internal.h"
+#include "episdet_lib.h"
+
+namespace epi {
+
+using Clock = std::chrono::steady_clock;
+
+static double ms_between(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<uint8_t> geno, pheno;
+    uint32_t N = 0, M = 0;
+    uint64_t footprint = 0;
+    Clock::time_point submitted, admitted;
+    bool cancel = false;
+    bool done = false;
+    bool charged = false;  // footprint counted in bytes_in_use
+    int status = EPI_OK;
+    epi_cohort *co = nullptr;
+    Job job;
+    epi_result result;
+};
+
+// Work units of a row: (items of the row) x (64-bit words of both classes);
+// the scan kernel's time per item is proportional to the words it loops over.
+static double row_units(const Row &r, uint32_t nw, const epi_cohort *co)
+{
+    const double items = (double)((r.icount + nw - 1) / nw);
+    return items * (double)(co->W[0] + co->W[1] + 2);
+}
+
+}  // namespace epi
+
+using namespace epi;
+
+struct epi_server {
+    epi_ctx *ctx = nullptr;
+    epi_server_config cfg;
+    ScanConfig scfg;
+    LaunchPlan plan;
+    std::thread worker;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::deque<Request *> pending;
+    std::vector<Request *> active;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    epi_server_stats stats;
+    double units_per_ms = 1e5;  // throughput estimate, refined by every round
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_sig;
+
+    void run();
+    int round(std::vector<Request *> &act);
+    int admit_locked_ctx(Request *r);
+    void release_locked_ctx(Request *r);
+    void complete(Request *r, int status);
+};
+
+void epi_server::complete(Request *r, int status)
+{
+    std::lock_guard<std::mutex> lk(m);
+    r->status = status;
+    r->done = true;
+    r->geno.clear();
+    r->geno.shrink_to_fit();
+    r->pheno.clear();
+    r->pheno.shrink_to_fit();
+    if (status == EPI_OK)
+        stats.completed++;
+    else if (status == EPI_ECANCELED)
+        stats.cancelled++;
+    else
+        stats.failed++;
+    if (r->charged) {
+        stats.bytes_in_use -= std::min(stats.bytes_in_use, r->footprint);
+        r->charged = false;
+    }
+    cv_done.notify_all();
+}
+
+// ctx->mu held.
+int epi_server::admit_locked_ctx(Request *r)
+{
+    int st = cohort_create_locked(ctx, r->geno.data(), r->pheno.data(), r->N, r->M, &r->co);
+    if (st == EPI_OK)
+        st = job_init(ctx, r->job, r->co, scfg);
+    if (st == EPI_OK)
+        st = enqueue_pairs(ctx, r->co);
+    r->admitted = Clock::now();
+    return st;
+}
+
+// ctx->mu held; the stream has no work left that reads the cohort.
+void epi_server::release_locked_ctx(Request *r)
+{
+    if (r->co) {
+        delete r->co;
+        r->co = nullptr;
+    }
+    r->job.drows.release();
+    r->job.list.release();
+    r->job.rows.clear();
+    r->job.rows.shrink_to_fit();
+}
+
+// One scheduling round over `act` (ctx->mu held).  Returns the device time.
+int epi_server::round(std::vector<Request *> &act)
+{
+    const uint32_t cap = cfg.max_active;
+    const uint32_t n = (uint32_t)act.size();
+    // fair share of the round budget per request, at least one row each
+    const double budget = std::max(units_per_ms, 1.0) * cfg.round_ms;
+    const double share = budget / std::max<uint32_t>(n, 1);
+    std::vector<Job *> jp(n);
+    std::vector<std::pair<size_t, size_t>> ranges(n);
+    double units = 0.0;
+    for (uint32_t b = 0; b < n; ++b) {
+        Job &J = act[b]->job;
+        jp[b] = &J;
+        size_t r = J.next_row;
+        double u = 0.0;
+        while (r < J.rows.size() && (r == J.next_row || u + row_units(J.rows[r], J.nw, J.co) <= share)) {
+            u += row_units(J.rows[r], J.nw, J.co);
+            ++r;
+        }
+        ranges[b] = std::make_pair(J.next_row, r);
+        units += u;
+    }
+    std::vector<unsigned char> host;
+    RoundLayout L;
+    uint64_t items = 0;
+    EPI_TRY(build_round(jp.data(), ranges.data(), n, cap, host, L, items));
+    EPI_TRY(reserve_round(ctx, plan, cap, L.total));
+    EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
+    std::vector<uint64_t> sig = {(uint64_t)cap, (uint64_t)plan.grid, L.total,
+                                 (uint64_t)(uintptr_t)ctx->round.ptr,
+                                 (uint64_t)(uintptr_t)ctx->counters.ptr,
+                                 (uint64_t)(uintptr_t)ctx->slots.ptr,
+                                 (uint64_t)(uintptr_t)ctx->t32.ptr};
+    if (!graph || sig != graph_sig) {
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        EPI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_round_kernels(ctx, plan, cap, L, true, false, nullptr, nullptr);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (st != EPI_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return st;
+        }
+        EPI_HIP(ce);
+        hipError_t ie = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        EPI_HIP(ie);
+        graph_sig = sig;
+    }
+    EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    EPI_HIP(hipGraphLaunch(graph, ctx->stream));
+    EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    EPI_HIP(hipEventSynchronize(ctx->ev1));
+    ctx->stage_pending = false;
+    float dms = 0.0f;
+    EPI_HIP(hipEventElapsedTime(&dms, ctx->ev0, ctx->ev1));
+    if (dms > 0.05f && units > 0.0) {
+        const double rate = units / dms;
+        units_per_ms = 0.7 * units_per_ms + 0.3 * rate;
+    }
+    std::lock_guard<std::mutex> lk(m);
+    stats.rounds++;
+    stats.device_ms += dms;
+    return EPI_OK;
+}
+
+void epi_server::run()
+{
+    for (;;) {
+        // ---- admission (FIFO, by device footprint) ----
+        std::vector<Request *> admit, cancelled;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+            if (stop && pending.empty() && active.empty())
+                return;
+            while (!pending.empty()) {
+                Request *r = pending.front();
+                if (r->cancel || stop) {
+                    pending.pop_front();
+                    cancelled.push_back(r);
+                    continue;
+                }
+                if (active.size() + admit.size() >= cfg.max_active ||
+                    stats.bytes_in_use + r->footprint > cfg.memory_budget)
+                    break;
+                pending.pop_front();
+                stats.bytes_in_use += r->footprint;
+                r->charged = true;
+                admit.push_back(r);
+            }
+            stats.pending = (uint32_t)pending.size();
+        }
+        for (Request *r : cancelled)
+            complete(r, EPI_ECANCELED);
+
+        std::vector<Request *> finished, failed;
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            if (hipSetDevice(ctx->device) != hipSuccess)
+                break;
+            for (Request *r : admit) {
+                const int st = admit_locked_ctx(r);
+                if (st != EPI_OK) {
+                    (void)hipStreamSynchronize(ctx->stream);
+                    ctx->stage_pending = false;
+                    release_locked_ctx(r);
+                    r->status = st;
+                    failed.push_back(r);
+                    continue;
+                }
+                std::lock_guard<std::mutex> lk(m);
+                active.push_back(r);
+            }
+            std::vector<Request *> act;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                stats.active = (uint32_t)active.size();
+                for (Request *r : active)
+                    if (!(r->cancel || stop))
+                        act.push_back(r);
+            }
+            int st = act.empty() ? EPI_OK : round(act);
+            // requests that are cancelled, finished, or hit by a failed round
+            std::vector<Request *> keep, done_ok, drop;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                for (Request *r : active) {
+                    if (r->cancel || stop)
+                        drop.push_back(r);
+                    else if (st != EPI_OK)
+                        drop.push_back(r);
+                    else if (r->job.next_row >= r->job.rows.size())
+                        done_ok.push_back(r);
+                    else
+                        keep.push_back(r);
+                }
+                active = keep;
+                stats.active = (uint32_t)active.size();
+            }
+            if (!done_ok.empty()) {
+                std::vector<Job *> jp;
+                for (Request *r : done_ok)
+                    jp.push_back(&r->job);
+                std::vector<epi_result> res(done_ok.size());
+                const int fs = finish_jobs(ctx, jp.data(), (uint32_t)jp.size(), res.data(), 0.0, 0.0);
+                const auto now = Clock::now();
+                for (size_t b = 0; b < done_ok.size(); ++b) {
+                    Request *r = done_ok[b];
+                    r->result = res[b];
+                    r->result.latency_ms = ms_between(r->submitted, now);
+                    r->result.queue_ms = ms_between(r->submitted, r->admitted);
+                    r->result.scan_ms = ms_between(r->admitted, now);
+                    r->status = fs;
+                    finished.push_back(r);
+                }
+            }
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->stage_pending = false;
+            for (Request *r : finished)
+                release_locked_ctx(r);
+            for (Request *r : drop) {
+                release_locked_ctx(r);
+                r->status = (r->cancel || stop) ? EPI_ECANCELED : st;
+                failed.push_back(r);
+            }
+        }
+        for (Request *r : finished)
+            complete(r, r->status);
+        for (Request *r : failed)
+            complete(r, r->status);
+    }
+}
+
+extern "C" int epi_server_create(epi_ctx *ctx, const epi_server_config *config,
+                                 epi_server **out)
+{
+    if (!ctx || !out)
+        return EPI_EINVAL;
+    *out = nullptr;
+    epi_server_config cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    if (config)
+        cfg = *config;
+    if (cfg.max_active == 0)
+        cfg.max_active = 32;
+    if (cfg.max_active > 64)
+        return EPI_EINVAL;
+    if (cfg.max_samples == 0)
+        cfg.max_samples = 32768;
+    if (cfg.round_ms <= 0.0)
+        cfg.round_ms = 20.0;
+    std::unique_ptr<epi_server> s(new (std::nothrow) epi_server());
+    if (!s)
+        return EPI_ENOMEM;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        EPI_HIP(hipSetDevice(ctx->device));
+        if (cfg.memory_budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            EPI_HIP(hipMemGetInfo(&free_b, &total_b));
+            cfg.memory_budget = total_b / 4;
+        }
+        EPI_TRY(ctx_ensure_tables(ctx, cfg.max_samples));
+        s->scfg = resolve_config(ctx, nullptr, cfg.max_samples);
+        EPI_TRY(plan_launch(ctx, s->scfg, cfg.max_samples + 2, false, s->plan));
+    }
+    s->ctx = ctx;
+    s->cfg = cfg;
+    std::memset(&s->stats, 0, sizeof(s->stats));
+    epi_server *raw = s.get();
+    try {
+        s->worker = std::thread([raw] { raw->run(); });
+    } catch (...) {
+        return EPI_ENOMEM;
+    }
+    *out = s.release();
+    return EPI_OK;
+}
+
+extern "C" int epi_server_destroy(epi_server *s)
+{
+    if (!s)
+        return EPI_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable())
+        s->worker.join();
+    if (s->graph) {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        (void)hipGraphExecDestroy(s->graph);
+    }
+    delete s;
+    return EPI_OK;
+}
+
+extern "C" int epi_submit(epi_server *s, const uint8_t *geno, const uint8_t *pheno,
+                          uint32_t n_samples, uint32_t n_snps, uint64_t *request_id)
+{
+    if (!s || !geno || !pheno || !request_id || !cohort_shape_ok(n_samples, n_snps) ||
+        n_samples > s->cfg.max_samples)
+        return EPI_EINVAL;
+    const uint64_t fp = cohort_footprint(n_samples, n_snps);
+    if (fp > s->cfg.memory_budget)
+        return EPI_ENOMEM;
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r)
+        return EPI_ENOMEM;
+    try {
+        r->geno.assign(geno, geno + (size_t)n_samples * n_snps);
+        r->pheno.assign(pheno, pheno + n_samples);
+    } catch (...) {
+        return EPI_ENOMEM;
+    }
+    r->N = n_samples;
+    r->M = n_snps;
+    r->footprint = fp;
+    r->submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        if (s->stop)
+            return EPI_EBUSY;
+        r->id = s->next_id++;
+        *request_id = r->id;
+        s->pending.push_back(r.get());
+        s->stats.submitted++;
+        s->stats.pending = (uint32_t)s->pending.size();
+        s->all.emplace(r->id, std::move(r));
+    }
+    s->cv_work.notify_all();
+    return EPI_OK;
+}
+
+extern "C" int epi_poll(epi_server *s, uint64_t id, int32_t timeout_ms, epi_result *result)
+{
+    if (!s)
+        return EPI_EINVAL;
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->all.find(id);
+    if (it == s->all.end())
+        return EPI_ENOENT;
+    Request *r = it->second.get();
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0)
+        s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return EPI_EAGAIN;
+    const int st = r->status;
+    if (st == EPI_OK && result)
+        *result = r->result;
+    s->all.erase(it);
+    return st;
+}
+
+extern "C" int epi_cancel(epi_server *s, uint64_t id)
+{
+    if (!s)
+        return EPI_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        auto it = s->all.find(id);
+        if (it == s->all.end())
+            return EPI_ENOENT;
+        it->second->cancel = true;
+    }
+    s->cv_work.notify_all();
+    return EPI_OK;
+}
+
+extern "C" int epi_server_get_stats(epi_server *s, epi_server_stats *out)
+{
+    if (!s || !out)
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->stats;
+    return EPI_OK;
+}

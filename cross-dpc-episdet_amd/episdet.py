@@ -53,6 +53,8 @@ class EpiResult(ctypes.Structure):
         ("n_controls", _u32), ("n_cases", _u32),
         ("scan_ms", ctypes.c_double),
         ("kernel_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("queue_ms", ctypes.c_double),
         ("max_abs_err_fp32", ctypes.c_double),
         ("fp64_order_differs", ctypes.c_int32),
         ("n_top", _u32),
@@ -62,6 +64,18 @@ class EpiResult(ctypes.Structure):
 
 class EpiTile(ctypes.Structure):
     _fields_ = [("variant", _u32), ("snps_j", _u32), ("waves", _u32), ("words", _u32)]
+
+
+class EpiServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("max_active", _u32),
+                ("max_samples", _u32), ("round_ms", ctypes.c_double)]
+
+
+class EpiServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("rounds", ctypes.c_uint64), ("pending", _u32), ("active", _u32),
+                ("bytes_in_use", ctypes.c_uint64), ("device_ms", ctypes.c_double)]
 
 
 # Every entry point of include/episdet.h: (name, restype, argtypes).
@@ -80,6 +94,17 @@ _PROTOS = [
                                       ctypes.POINTER(EpiResult)]),
     ("epi_count", ctypes.c_int, [_VP, _VP, _VP, _u32, _VP, _VP, _VP]),
     ("epi_scan_dump", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), _VP, _VP]),
+    ("epi_autotune", ctypes.c_int, [_VP, _u32, _u32, ctypes.POINTER(EpiTile),
+                                    ctypes.POINTER(ctypes.c_double)]),
+    ("epi_tune_set", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), _u32]),
+    ("epi_tune_get", ctypes.c_int, [_VP, _u32, ctypes.POINTER(EpiTile)]),
+    ("epi_server_create", ctypes.c_int, [_VP, ctypes.POINTER(EpiServerConfig),
+                                         ctypes.POINTER(_VP)]),
+    ("epi_server_destroy", ctypes.c_int, [_VP]),
+    ("epi_submit", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(ctypes.c_uint64)]),
+    ("epi_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(EpiResult)]),
+    ("epi_cancel", ctypes.c_int, [_VP, ctypes.c_uint64]),
+    ("epi_server_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(EpiServerStats)]),
     ("epi_generate_reference", ctypes.c_int, [_u32, _u32, _VP, _VP]),
     ("epi_generate_hwe", ctypes.c_int, [ctypes.c_uint64, _u32, _u32, ctypes.c_double,
                                         ctypes.c_double, _u32, _VP, _VP]),
@@ -192,6 +217,8 @@ class ScanResult:
     n_cases: int
     scan_ms: float
     kernel_ms: float
+    latency_ms: float
+    queue_ms: float
     max_abs_err_fp32: float
     fp64_order_differs: bool
     top: List[Candidate] = field(default_factory=list)
@@ -209,7 +236,8 @@ class ScanResult:
                                  np.ctypeslib.as_array(c.counts).copy()))
         return cls(r.i, r.j, r.k, float(r.score), float(r.score_fp64), int(r.triplets),
                    r.n_samples, r.n_snps, r.n_controls, r.n_cases, float(r.scan_ms),
-                   float(r.kernel_ms), float(r.max_abs_err_fp32), bool(r.fp64_order_differs), top)
+                   float(r.kernel_ms), float(r.latency_ms), float(r.queue_ms),
+                   float(r.max_abs_err_fp32), bool(r.fp64_order_differs), top)
 
 
 # ------------------------------------------------------------------ objects
@@ -267,6 +295,30 @@ class Context:
                                _ptr(s64)), "epi_count")
         return counts, s32, s64
 
+    def autotune(self, n_samples: int, n_snps: int):
+        t = EpiTile()
+        ms = ctypes.c_double()
+        _check(lib().epi_autotune(self.handle, n_samples, n_snps, ctypes.byref(t), ctypes.byref(ms)),
+               "epi_autotune")
+        return {"snps_j": t.snps_j, "waves": t.waves}, float(ms.value)
+
+    def tune_set(self, table):
+        """table: [(max_samples, {"snps_j": J, "waves": W}), ...] in increasing order."""
+        n = len(table)
+        ms = np.array([int(m) for m, _ in table], np.uint32)
+        tiles = (EpiTile * max(n, 1))()
+        for e, (_, t) in enumerate(table):
+            tiles[e] = _tile(t)
+        _check(lib().epi_tune_set(self.handle, _ptr(ms), tiles, n), "epi_tune_set")
+
+    def tune_get(self, n_samples: int):
+        t = EpiTile()
+        _check(lib().epi_tune_get(self.handle, n_samples, ctypes.byref(t)), "epi_tune_get")
+        return {"snps_j": t.snps_j, "waves": t.waves}
+
+    def server(self, **config) -> "Server":
+        return Server(self, **config)
+
     def scan_dump(self, cohort: "Cohort", tile=None, with_counts: bool = False):
         m = cohort.n_snps
         nt = m * (m - 1) * (m - 2) // 6
@@ -302,6 +354,54 @@ class Cohort:
     def close(self) -> None:
         if self.handle:
             lib().epi_cohort_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching request loop (epi_server_*)."""
+
+    def __init__(self, ctx: Context, memory_budget: int = 0, max_active: int = 0,
+                 max_samples: int = 0, round_ms: float = 0.0):
+        cfg = EpiServerConfig(memory_budget, max_active, max_samples, round_ms)
+        h = ctypes.c_void_p()
+        _check(lib().epi_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)),
+               "epi_server_create")
+        self.handle = h
+
+    def submit(self, geno: np.ndarray, pheno: np.ndarray) -> int:
+        geno = np.ascontiguousarray(geno, dtype=np.uint8)
+        pheno = np.ascontiguousarray(pheno, dtype=np.uint8)
+        rid = ctypes.c_uint64()
+        _check(lib().epi_submit(self.handle, _ptr(geno), _ptr(pheno), geno.shape[0], geno.shape[1],
+                                ctypes.byref(rid)), "epi_submit")
+        return int(rid.value)
+
+    def poll(self, rid: int, timeout_ms: int = -1) -> Optional[ScanResult]:
+        """Result of request `rid`; None if not ready within timeout_ms."""
+        r = EpiResult()
+        code = lib().epi_poll(self.handle, rid, timeout_ms, ctypes.byref(r))
+        if code == EPI_EAGAIN:
+            return None
+        _check(code, f"epi_poll({rid})")
+        return ScanResult.from_c(r)
+
+    def cancel(self, rid: int) -> None:
+        _check(lib().epi_cancel(self.handle, rid), f"epi_cancel({rid})")
+
+    def stats(self) -> dict:
+        st = EpiServerStats()
+        _check(lib().epi_server_get_stats(self.handle, ctypes.byref(st)), "epi_server_get_stats")
+        return {k: getattr(st, k) for k, _ in EpiServerStats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            lib().epi_server_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -71,7 +71,9 @@ typedef struct {
     uint32_t n_samples, n_snps;
     uint32_t n_controls, n_cases;
     double scan_ms;           /* host wall time, launch to result on host */
-    double kernel_ms;         /* device time of the scan kernel */
+    double kernel_ms;         /* device time of the scan kernel(s) */
+    double latency_ms;        /* server requests: submission to result ready */
+    double queue_ms;          /* server requests: submission to admission */
     double max_abs_err_fp32;  /* max |score - score_fp64| over top[] */
     int32_t fp64_order_differs; /* 1 if FP64 ranks top[] differently */
     uint32_t n_top;           /* entries of top[] that are filled */
@@ -133,6 +135,57 @@ int epi_count(epi_ctx *ctx, epi_cohort *cohort, const uint32_t *triplets,
  * n_snps <= 512. */
 int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
                   float *scores, uint32_t *counts);
+
+/* Tile autotuning ---------------------------------------------------------- */
+/* Times every supported tile shape on a synthetic Hardy-Weinberg cohort of
+ * n_samples x min(n_snps, 768) and returns the fastest (no reference
+ * counterpart; the reference takes block_size from the command line,
+ * GPU/v4.cpp:212).  best_ms may be NULL. */
+int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, epi_tile *best,
+                 double *best_ms);
+/* Tile table used when epi_scan gets no explicit tile: entry e applies to
+ * n_samples <= max_samples[e] (entries in increasing order).  epi_open
+ * loads tune/<arch>.txt next to the library's directory when it exists
+ * (lines "max_samples snps_j waves"). */
+int epi_tune_set(epi_ctx *ctx, const uint32_t *max_samples, const epi_tile *tiles,
+                 uint32_t n);
+int epi_tune_get(epi_ctx *ctx, uint32_t n_samples, epi_tile *tile);
+
+/* Serving ---------------------------------------------------------------- */
+/* A request loop on one context (no reference counterpart; the reference
+ * scans one generated dataset per process run, GPU/v4.cpp:198-624).
+ * Submitted cohorts are packed into HBM when their device footprint fits
+ * the budget (FIFO admission); every scheduling round is one persistent
+ * scan launch over a share of the remaining tiles of every admitted
+ * request, so requests that arrive while others run join at the next
+ * round.  Each result equals epi_scan of the same data bit for bit. */
+typedef struct {
+    uint64_t memory_budget;  /* device bytes for resident cohorts; 0: 1/4 of the device */
+    uint32_t max_active;     /* requests on the device at once (1..64); 0: 32 */
+    uint32_t max_samples;    /* largest n_samples accepted; 0: 32768 */
+    double round_ms;         /* target device time of one round; 0: 20 */
+} epi_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, cancelled, failed, rounds;
+    uint32_t pending, active;
+    uint64_t bytes_in_use;
+    double device_ms;        /* summed device time of all rounds */
+} epi_server_stats;
+
+int epi_server_create(epi_ctx *ctx, const epi_server_config *config, epi_server **out);
+/* Cancels what is still queued or running, then stops the request loop. */
+int epi_server_destroy(epi_server *server);
+/* Copies the inputs (they may be freed on return) and queues a scan. */
+int epi_submit(epi_server *server, const uint8_t *geno, const uint8_t *pheno,
+               uint32_t n_samples, uint32_t n_snps, uint64_t *request_id);
+/* Waits up to timeout_ms (< 0: forever) for a request.  On success the
+ * result is copied out and the request is forgotten.  EPI_EAGAIN: still
+ * running; EPI_ECANCELED / EPI_ENOMEM / other codes: the request failed. */
+int epi_poll(epi_server *server, uint64_t request_id, int32_t timeout_ms,
+             epi_result *result);
+int epi_cancel(epi_server *server, uint64_t request_id);
+int epi_server_get_stats(epi_server *server, epi_server_stats *stats);
 
 /* Data generators ------------------------------------------------------- */
 /* The reference generator: srand(100), rand()%3, N/2 cases

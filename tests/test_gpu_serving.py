@@ -1,0 +1,146 @@
+"""Batched scans (graph-captured, replayed), the continuous-batching server
+and the tile autotuner, all through the C ABI.  Every result must equal a
+plain scan of the same cohort bit for bit."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    return ((a.i, a.j, a.k) == (b.i, b.j, b.k) and
+            [(c.i, c.j, c.k, c.score, c.score_fp64) for c in a.top] ==
+            [(c.i, c.j, c.k, c.score, c.score_fp64) for c in b.top])
+
+
+def make_cohorts(epi, ctx, n, seed, m_range=(20, 300), n_range=(100, 4000)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n):
+        M = int(rng.integers(*m_range))
+        N = int(rng.integers(*n_range))
+        g, p = epi.generate_hwe(seed * 100 + b, N, M, int(round(N * rng.uniform(0.3, 0.7))))
+        out.append((g, p))
+    return out
+
+
+def test_batch_graph_replay(epi, ctx):
+    """Same batch twice (the second call replays the captured graph), then a
+    batch of other cohorts with the same shape, then a different shape."""
+    data = make_cohorts(epi, ctx, 12, 5)
+    cos = [ctx.cohort(g, p) for g, p in data]
+    try:
+        single = [ctx.scan(c) for c in cos]
+        for rep in range(3):
+            batch = ctx.scan_batch(cos)
+            assert all(same(a, b) for a, b in zip(single, batch)), rep
+        rev = ctx.scan_batch(cos[::-1])
+        assert all(same(a, b) for a, b in zip(single[::-1], rev))
+        part = ctx.scan_batch(cos[3:7])
+        assert all(same(a, b) for a, b in zip(single[3:7], part))
+    finally:
+        for c in cos:
+            c.close()
+
+
+def test_server_results_equal_plain_scans(epi, ctx):
+    data = make_cohorts(epi, ctx, 16, 9, (10, 400), (64, 6000))
+    plain = []
+    for g, p in data:
+        with ctx.cohort(g, p) as co:
+            plain.append(ctx.scan(co))
+    with ctx.server(max_active=6, round_ms=2.0) as srv:
+        ids = [srv.submit(g, p) for g, p in data]
+        got = [srv.poll(i, 120000) for i in ids]
+        st = srv.stats()
+    say(st)
+    assert st["completed"] == len(data) and st["failed"] == 0
+    assert st["rounds"] > 1 and st["bytes_in_use"] == 0
+    for a, b in zip(plain, got):
+        assert b is not None and same(a, b)
+        assert b.latency_ms >= b.queue_ms >= 0
+
+
+def test_server_late_arrivals_join_running_work(epi, ctx):
+    """A large request is running when small ones arrive; the small ones
+    complete before it (continuous batching) and all results are exact."""
+    g, p = epi.generate_hwe(71, 16384, 2048, 8192)
+    small = make_cohorts(epi, ctx, 4, 13, (20, 60), (200, 800))
+    with ctx.cohort(g, p) as co:
+        big_plain = ctx.scan(co)
+    say("big plain scan", big_plain.scan_ms, "ms")
+    with ctx.server(max_active=8, round_ms=5.0) as srv:
+        t_big = time.perf_counter()
+        big = srv.submit(g, p)
+        time.sleep(0.05)
+        t_small = time.perf_counter()
+        sids = [srv.submit(a, b) for a, b in small]
+        sres = [srv.poll(i, 120000) for i in sids]
+        bres = srv.poll(big, 300000)
+    assert same(big_plain, bres)
+    done_big = t_big + bres.latency_ms / 1e3
+    for r in sres:
+        assert t_small + r.latency_ms / 1e3 < done_big, (r.latency_ms, bres.latency_ms)
+    for (a, b), r in zip(small, sres):
+        with ctx.cohort(a, b) as co:
+            assert same(ctx.scan(co), r)
+
+
+def test_server_memory_admission_and_cancel(epi, ctx):
+    data = make_cohorts(epi, ctx, 6, 17, (200, 300), (1000, 2000))
+    # budget for about one cohort at a time: requests are admitted one by one
+    with ctx.cohort(*data[0]) as co:
+        one = co.device_bytes
+    with ctx.server(memory_budget=int(one * 1.6), max_active=8, round_ms=2.0) as srv:
+        ids = [srv.submit(g, p) for g, p in data]
+        srv.cancel(ids[-1])
+        res = []
+        for i in ids[:-1]:
+            res.append(srv.poll(i, 120000))
+        with pytest.raises(epi.EpiError) as e:
+            srv.poll(ids[-1], 120000)
+        assert e.value.code == epi.EPI_ECANCELED
+        st = srv.stats()
+        # a cohort larger than the whole budget is refused at submission
+        g, p = epi.generate_hwe(3, 4000, 2000, 2000)
+        with pytest.raises(epi.EpiError) as e:
+            srv.submit(g, p)
+        assert e.value.code == epi.EPI_ENOMEM
+    assert st["completed"] == 5 and st["cancelled"] == 1
+    for (g, p), r in zip(data, res):
+        with ctx.cohort(g, p) as co:
+            assert same(ctx.scan(co), r)
+
+
+def test_server_poll_unknown_and_timeout(epi, ctx):
+    with ctx.server() as srv:
+        with pytest.raises(epi.EpiError) as e:
+            srv.poll(12345, 0)
+        assert e.value.code == epi.EPI_ENOENT
+        g, p = epi.generate_hwe(1, 4096, 800, 2048)
+        rid = srv.submit(g, p)
+        r = srv.poll(rid, 0)  # almost certainly not finished yet
+        if r is None:
+            r = srv.poll(rid, 120000)
+        assert r is not None and r.n_snps == 800
+
+
+def test_autotune_and_tile_table(epi, ctx):
+    tile, ms = ctx.autotune(4096, 400)
+    say("autotune 4096 x 400:", tile, ms)
+    assert tile["snps_j"] in (1, 2, 4, 8) and tile["waves"] in (4, 8, 16) and ms > 0
+    ctx.tune_set([(2048, {"snps_j": 2, "waves": 4}), (1 << 20, tile)])
+    assert ctx.tune_get(1000) == {"snps_j": 2, "waves": 4}
+    assert ctx.tune_get(5000) == tile
+    g, p = epi.generate_hwe(2, 1500, 100, 700)
+    with ctx.cohort(g, p) as co:
+        a = ctx.scan(co)
+        b = ctx.scan(co, tile={"snps_j": 8, "waves": 16})
+    assert same(a, b)
+    ctx.tune_set([])

@@ -134,6 +134,22 @@ def test_log_table_against_lgamma(orc):
     assert T[0] == 0 and T[1] == 0 and T[2] == np.float32(math.log(2))
 
 
+def test_log_table_f64_against_lgamma(orc):
+    """FP64 twin (used by the top-64 rescoring): exact log sums below 748,
+    the Stirling form's truncation error only (no float rounding) above."""
+    T = orc.log_table_f64(70000)
+    lg = np.array([math.lgamma(x + 1) for x in range(70001)])
+    err = np.abs(T - lg)
+    assert err[:748].max() < 1e-9
+    # (n + 1/2) ln n - (n - 1) - 0.0810 - ln n! = 0.9190 - ln(2 pi)/2 - 1/(12 n)
+    # + 1/(360 n^3) - ... (Stirling series)
+    n = np.arange(748, 70001, dtype=np.float64)
+    series = 0.919 - 0.5 * math.log(2 * math.pi) - 1 / (12 * n) + 1 / (360 * n ** 3)
+    assert np.abs((T[748:] - lg[748:]) - series).max() < 1e-8
+    T32 = orc.log_table(70000)
+    assert np.array_equal(T32[748:], T[748:].astype(np.float32))
+
+
 def test_log_table_fixture_bits(orc, golden):
     fx = golden("log_table.json")
     for n, digest in fx["sha256_float32_le"].items():
