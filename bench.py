@@ -8,9 +8,22 @@ triplets.  Prints one JSON line with the metric, the roofline of the scan
 kernel and the reference CPU baseline timed on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                  [--no-batch] [--no-serve]
+
+Two more legs are reported in the same line (skippable):
+  batched: 64 cohorts from one seed (SNPs ~ U[128,1024], samples ~
+    U[512,8192], case fraction ~ U[0.3,0.7]) in one graph-captured
+    persistent launch, against 64 separate plain scans; per-cohort results
+    must be identical.
+  serving: Poisson-arriving requests (SNPs log-U[64,2048], samples
+    log-U[256,16384], case fraction U[0.3,0.7]) at an offered load of 0.8 x
+    the measured single-scan throughput, through the continuous-batching
+    server, against one-request-at-a-time plain scans under the same
+    arrival times; p50/p99 latency for both, identical results.
 """
 import argparse
 import importlib.util
+import threading
 import json
 import math
 import os
@@ -76,6 +89,114 @@ def cpu_baseline(n_samples, n_snps_sample):
                       f"M={n_snps_sample} (reference generator), scan {secs:.2f} s"}
 
 
+def same_result(a, b):
+    return ((a.i, a.j, a.k) == (b.i, b.j, b.k) and
+            [(c.i, c.j, c.k, c.score) for c in a.top] == [(c.i, c.j, c.k, c.score) for c in b.top])
+
+
+def trip(m):
+    return m * (m - 1) * (m - 2) // 6
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, max(0, math.ceil(q * len(xs)) - 1))]
+
+
+def batched_leg(epi, ctx, seed=4, n=64):
+    """64 uneven cohorts: one graph-captured launch vs 64 plain scans."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cohorts, work = [], 0
+    for b in range(n):
+        M = int(rng.integers(128, 1025))
+        N = int(rng.integers(512, 8193))
+        cases = int(round(N * rng.uniform(0.3, 0.7)))
+        g, p = epi.generate_hwe(seed * 1000 + b, N, M, cases)
+        cohorts.append(ctx.cohort(g, p))
+        work += trip(M) * N
+    try:
+        plain, t_plain = [], 0.0
+        for c in cohorts:
+            t0 = time.perf_counter()
+            plain.append(ctx.scan(c))
+            t_plain += time.perf_counter() - t0
+        ctx.scan_batch(cohorts)  # capture + first replay
+        reps, t0 = 3, time.perf_counter()
+        for _ in range(reps):
+            batch = ctx.scan_batch(cohorts)
+        t_batch = (time.perf_counter() - t0) / reps
+    finally:
+        for c in cohorts:
+            c.close()
+    ok = all(same_result(a, b) for a, b in zip(plain, batch))
+    return {"cohorts": n, "triplets_x_samples": work,
+            "value": work / t_batch, "ms": t_batch * 1e3,
+            "sequential_value": work / t_plain, "sequential_ms": t_plain * 1e3,
+            "speedup_vs_sequential": t_plain / t_batch, "identical_to_plain": ok}
+
+
+def serving_leg(epi, ctx, capacity, seed=5, n=64, load=0.8):
+    """Poisson stream through the server vs FIFO plain scans."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for r in range(n):
+        M = int(round(math.exp(rng.uniform(math.log(64), math.log(2048)))))
+        N = int(round(math.exp(rng.uniform(math.log(256), math.log(16384)))))
+        cases = int(round(N * rng.uniform(0.3, 0.7)))
+        g, p = epi.generate_hwe(seed * 1000 + r, N, M, cases)
+        reqs.append((g, p, trip(M) * N))
+    mean_work = sum(w for _, _, w in reqs) / n
+    lam = load * capacity / mean_work  # requests per second
+    gaps = rng.exponential(1.0 / lam, size=n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    # served: submit at the arrival times from a separate thread
+    results = [None] * n
+    with ctx.server(max_active=32, max_samples=16384, round_ms=10.0) as srv:
+        ids = [None] * n
+        t0 = time.perf_counter()
+
+        def feeder():
+            for r in range(n):
+                delay = arrivals[r] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[r] = srv.submit(reqs[r][0], reqs[r][1])
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        for r in range(n):
+            results[r] = srv.poll(ids[r], 600000)
+        t_end = time.perf_counter() - t0
+        st = srv.stats()
+    lat = [res.latency_ms for res in results]
+    # plain one-at-a-time baseline: real service times, FIFO queue on the
+    # same arrival times
+    service, plain = [], []
+    for g, p, _ in reqs:
+        with ctx.cohort(g, p) as co:
+            t1 = time.perf_counter()
+            plain.append(ctx.scan(co))
+            service.append(time.perf_counter() - t1)
+    done, lat_plain = 0.0, []
+    for r in range(n):
+        done = max(done, arrivals[r]) + service[r]
+        lat_plain.append((done - arrivals[r]) * 1e3)
+    total = sum(w for _, _, w in reqs)
+    ok = all(same_result(a, b) for a, b in zip(plain, results))
+    return {"requests": n, "offered_load": load, "arrival_rate_per_s": lam,
+            "triplets_x_samples": total,
+            "value": total / t_end, "makespan_ms": t_end * 1e3,
+            "device_busy_frac": st["device_ms"] / (t_end * 1e3),
+            "rounds": st["rounds"],
+            "latency_ms": {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)},
+            "plain_fifo": {"value": total / done, "makespan_ms": done * 1e3,
+                           "latency_ms": {"p50": pct(lat_plain, 0.5), "p99": pct(lat_plain, 0.99)}},
+            "identical_to_plain": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +209,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-snps", type=int, default=1024)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -112,11 +235,22 @@ def main():
                 kms.append(r.kernel_ms)
                 log(f"step {s}: {r.scan_ms:.1f} ms (scan kernel {r.kernel_ms:.1f} ms)")
             elapsed = time.perf_counter() - t0
-            device = ctx.device_name
-    trip = M * (M - 1) * (M - 2) // 6
-    value = trip * N * args.steps / elapsed
+        device = ctx.device_name
+        capacity = M * (M - 1) * (M - 2) // 6 * N * args.steps / elapsed
+        batched = serving = None
+        if not args.no_batch:
+            log("batched leg: 64 uneven cohorts")
+            batched = batched_leg(epi, ctx)
+            log(f"batched: {batched['ms']:.1f} ms vs {batched['sequential_ms']:.1f} ms sequential")
+        if not args.no_serve:
+            log("serving leg: Poisson stream")
+            serving = serving_leg(epi, ctx, capacity)
+            log(f"serving: p50 {serving['latency_ms']['p50']:.1f} ms, "
+                f"p99 {serving['latency_ms']['p99']:.1f} ms")
+    ntrip = trip(M)
+    value = ntrip * N * args.steps / elapsed
     kernel_s = statistics.mean(kms) / 1e3
-    ops = trip * (N / 32.0) * OPS_PER_TRIPLET_WORD
+    ops = ntrip * (N / 32.0) * OPS_PER_TRIPLET_WORD
     achieved = ops / kernel_s / 1e12
     peak = VALU_PEAK_OPS / 1e12
     traffic = None
@@ -130,9 +264,8 @@ def main():
             cpu = cpu_baseline(N, args.cpu_sample_snps)
         except Exception as e:  # reported, never fatal for the GPU number
             log(f"cpu baseline failed: {e}")
-    lat_sorted = sorted(lat)
     out = {
-        "metric": "triplets x samples/s at 4096 SNPs x 16384 samples",
+        "metric": "triplets x samples/s at 4096 SNPs x 16384 samples; p50/p99 request latency",
         "value": value,
         "unit": "triplets*samples/s",
         "n_gpus": 1,
@@ -146,18 +279,22 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"exhaustive 3-way K2 scan, {M} SNPs x {N} samples "
                                f"({cases} cases), HWE MAF U[0.05,0.5], seed {args.seed}",
-                   "snps": M, "samples": N, "cases": cases, "triplets": trip,
+                   "snps": M, "samples": N, "cases": cases, "triplets": ntrip,
                    "device": device},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "ops_per_unit": "54 int32 lane-ops per (triplet x 32 samples)",
                      "kernel_ms": kernel_s * 1e3},
-        "latency_ms": {"p50": lat_sorted[len(lat) // 2],
-                       "p99": lat_sorted[min(len(lat) - 1, math.ceil(0.99 * len(lat)) - 1)]},
+        # request latency of the serving stream when it ran, else of the scans above
+        "latency_ms": (serving["latency_ms"] if serving else
+                       {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)}),
+        "scan_latency_ms": {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)},
         "best": [r.i, r.j, r.k],
         "score": r.score,
         "max_abs_err_fp32": r.max_abs_err_fp32,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
