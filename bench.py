@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-snps", type=int, default=1024)
+    ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 VALU, 2 int8 MFMA")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the one-scan comparison of the VALU and MFMA kernels")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
@@ -221,7 +224,7 @@ def main():
     N, M, cases = args.samples, args.snps, args.samples // 2
     log(f"generating {N} x {M} HWE data (seed {args.seed})")
     g, p = epi.generate_hwe(args.seed, N, M, cases)
-    tile = {"snps_j": args.snps_j, "waves": args.waves}
+    tile = {"variant": args.variant, "snps_j": args.snps_j, "waves": args.waves}
     with epi.Context(0) as ctx:
         with ctx.cohort(g, p) as co:
             for w in range(args.warmup):
@@ -235,6 +238,20 @@ def main():
                 kms.append(r.kernel_ms)
                 log(f"step {s}: {r.scan_ms:.1f} ms (scan kernel {r.kernel_ms:.1f} ms)")
             elapsed = time.perf_counter() - t0
+            compare = None
+            if not args.no_compare:
+                # one scan with each formulation of the 8 popcounted cells
+                compare = {}
+                for name, var in (("valu", 1), ("mfma", 2)):
+                    rv = ctx.scan(co, {"variant": var})
+                    kern = rv.kernel_ms / 1e3
+                    compare[name] = {"kernel_ms": rv.kernel_ms, "scan_ms": rv.scan_ms,
+                                     "triplets_x_samples_per_s": trip(M) * N / kern,
+                                     "identical": same_result(rv, r)}
+                    log(f"{name}: kernel {rv.kernel_ms:.1f} ms")
+                # MFMA work: 8 int8 MACs per triplet-sample (2 ops each)
+                compare["mfma"]["int8_tops"] = trip(M) * N * 16 / (compare["mfma"]["kernel_ms"] / 1e3) / 1e12
+                compare["mfma"]["mfma_util"] = compare["mfma"]["int8_tops"] / 5000.0
         device = ctx.device_name
         capacity = M * (M - 1) * (M - 2) // 6 * N * args.steps / elapsed
         batched = serving = None
@@ -293,6 +310,7 @@ def main():
         "score": r.score,
         "max_abs_err_fp32": r.max_abs_err_fp32,
         "cpu_baseline": cpu,
+        "variants": compare,
         "batched": batched,
         "serving": serving,
     }

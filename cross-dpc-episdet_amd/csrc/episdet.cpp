@@ -216,6 +216,7 @@ epi_cohort::~epi_cohort()
         wm[c].release();
         pair[c].release();
         single[c].release();
+        E[c].release();
     }
     desc.release();
 }
@@ -332,6 +333,8 @@ CohortDev epi_cohort::host_desc() const
         h.wm[c] = (const uint4 *)wm[c].ptr;
         h.pair[c] = (uint4 *)pair[c].ptr;
         h.single[c] = (uint2 *)single[c].ptr;
+        h.E[c] = (const uint8_t *)E[c].ptr;
+        h.EL[c] = EL[c];
         h.W[c] = W[c];
         h.n[c] = n[c];
     }

@@ -36,6 +36,8 @@ struct CohortDev {
     const uint4 *wm[2];   // word-major [W[c]][Mp]   (per-lane loads)
     uint4 *pair[2];       // [Mp][Mp] for p < q: {n00, n01, n10, n11}
     uint2 *single[2];     // [Mp]: {count g0, count g1}
+    const uint8_t *E[2];  // MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
+    uint32_t EL[2];       // bytes per E row: 64 W[c] rounded up to 128, zero padded
     uint32_t W[2];        // 64-bit words per class
     uint32_t n[2];        // class sizes
     uint32_t M, Mp;       // SNPs, padded to a multiple of 64
@@ -79,12 +81,13 @@ struct ScanParams {
     const RoundHdr *hdr;
     const CohortDev *cohorts;  // [n_cohorts]
     const Segment *segs;       // [n_segments]
-    uint32_t *counters;        // [0] item queue head, [1 + c] slots used by cohort c
+    uint32_t *counters;        // [0] item queue head, [1 + c] candidates appended for cohort c
+    uint32_t *thr;             // [c] score threshold of cohort c (float bits)
     const float *table;        // ln(n!) FP32, table_len entries
     uint32_t table_len;
-    uint32_t slot_cap;         // slots per cohort
-    Cand *slots;               // [cohort][slot_cap][kTop]
-    uint32_t *overflow;        // set to 1 if a slot could not be stored
+    uint32_t slot_cap;         // candidate capacity per cohort
+    Cand *slots;               // [cohort][slot_cap] appended candidates
+    uint32_t *overflow;        // set to 1 if a candidate could not be stored
     float *dump_scores;        // dump mode: score per triplet rank (cohort 0)
     uint32_t *dump_counts;     // dump mode: 54 counts per triplet rank
 };
@@ -101,9 +104,10 @@ struct Rescored {
 
 // Scan kernel configuration (template instance selector).
 struct ScanConfig {
-    int snps_j;  // 1, 2, 4, 8
-    int waves;   // 4, 8, 16
-    int words;   // 1 or 2 sample words per loop step
+    int variant = 1;  // 1 VALU AND+popcount, 2 int8 MFMA
+    int snps_j = 4;   // VALU: 1, 2, 4, 8 j SNPs per wavefront; MFMA: 64 (j-block)
+    int waves = 8;    // wavefronts per workgroup
+    int words = 1;    // 64-bit sample words per loop step
 };
 
 // Launch wrappers (episdet_kernels.hip).
@@ -115,6 +119,14 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
 hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &host_cohort,
                         hipStream_t s);
 bool scan_config_supported(const ScanConfig &cfg);
+// MFMA variant (episdet_mfma.hip)
+hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, uint8_t *E,
+                         hipStream_t s);
+size_t mfma_lds_bytes(bool table_lds, uint32_t table_len);
+hipError_t mfma_occupancy(bool table_lds, bool dump, size_t lds_bytes, int *blocks_per_cu);
+hipError_t mfma_allow_lds(bool table_lds, bool dump, size_t lds_bytes);
+hipError_t launch_mfma_scan(bool table_lds, bool dump, const ScanParams &p, uint32_t grid,
+                            hipStream_t s);
 hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump,
                           size_t lds_bytes, int *blocks_per_cu);
 size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len);
@@ -127,6 +139,8 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s);
 hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
                         uint32_t slot_cap, Cand *const *lists, uint32_t max_cohorts,
                         hipStream_t s);
+hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *thr,
+                           uint32_t max_cohorts, hipStream_t s);
 // Rescores per_cohort candidates of each cohort: cands given either as one
 // array [n_cohorts][per_cohort] or (lists != nullptr) as per-cohort lists.
 hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands,

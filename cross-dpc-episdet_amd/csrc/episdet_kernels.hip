@@ -29,6 +29,7 @@
 
 #include "episdet_cells.h"
 #include "episdet_internal.h"
+#include "episdet_topk.h"
 
 #define EPI_CONST __attribute__((address_space(4)))
 
@@ -232,63 +233,10 @@ __device__ __forceinline__ void count8(const uint4 *X, const uint4 *Y,
     }
 }
 
-// Wave-level top-kTop: rank every buffered candidate against the others
-// (keys are unique) and keep the kTop best, sorted, in buf[0..kTop).
-__device__ __noinline__ void wave_compact(Cand *buf, Cand *keep, int &cnt, float &ts,
-                                          uint64_t &tk, uint32_t lane)
-{
-    wave_sync();
-    const int n = cnt;
-    for (int e = (int)lane; e < n; e += kLanes) {
-        const Cand me = buf[e];
-        int rank = 0;
-        for (int f = 0; f < n; ++f) {
-            const Cand o = buf[f];
-            rank += cand_less(o.s, o.key, me.s, me.key) ? 1 : 0;
-        }
-        if (rank < kTop)
-            keep[rank] = me;
-    }
-    wave_sync();
-    const int m = n < kTop ? n : kTop;
-    if ((int)lane < m)
-        buf[lane] = keep[lane];
-    cnt = m;
-    if (m == kTop) {
-        ts = keep[kTop - 1].s;
-        tk = keep[kTop - 1].key;
-    }
-    wave_sync();
-}
-
-__device__ __forceinline__ void wave_offer(bool valid, float s, uint64_t key, Cand *buf,
-                                           Cand *keep, int &cnt, float &ts, uint64_t &tk,
-                                           uint32_t lane)
-{
-    const bool pass = valid && cand_less(s, key, ts, tk);
-    const uint64_t mask = __ballot(pass);
-    if (mask == 0)
-        return;
-    const int nn = __popcll(mask);
-    if (cnt + nn > kWaveCap)
-        wave_compact(buf, keep, cnt, ts, tk, lane);
-    if (pass) {
-        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        Cand c;
-        c.s = s;
-        c.pad = 0;
-        c.key = key;
-        buf[cnt + below] = c;
-    }
-    cnt += nn;
-}
-
+// LDS: per-wavefront candidate buffers, then (optionally) the ln(n!) table.
 template <int NW>
 struct ScanLds {
     Cand cand[NW][kWaveCap + kTop];
-    uint32_t misc[4];
-    uint32_t wcnt[NW];
 };
 
 template <int NW>
@@ -297,62 +245,10 @@ __host__ __device__ constexpr size_t scan_lds_fixed()
     return (sizeof(ScanLds<NW>) + 15) & ~(size_t)15;
 }
 
-// Flush the workgroup's candidates for `cohort` into one slot of kTop
-// sorted entries.  Called by every thread of the workgroup.
-template <int NW>
-__device__ void wg_flush(const ScanParams &P, ScanLds<NW> &L, uint32_t cohort, int &cnt,
-                         float &ts, uint64_t &tk, uint32_t wave, uint32_t lane)
-{
-    Cand *buf = L.cand[wave];
-    Cand *keep = buf + kWaveCap;
-    if (cnt > kTop)
-        wave_compact(buf, keep, cnt, ts, tk, lane);
-    if (lane == 0)
-        L.wcnt[wave] = (uint32_t)cnt;
-    if (threadIdx.x == 0)
-        L.misc[1] = atomicAdd(&P.counters[1 + cohort], 1u);
-    __syncthreads();
-    const uint32_t slot = L.misc[1];
-    const uint32_t mine = L.wcnt[wave];
-    const bool have = lane < mine;
-    Cand me;
-    me.s = INFINITY;
-    me.pad = 0;
-    me.key = kEmptyKey;
-    if (have)
-        me = buf[lane];
-    uint32_t rank = 0, total = 0;
-    for (int w2 = 0; w2 < NW; ++w2) {
-        const uint32_t n2 = L.wcnt[w2];
-        total += n2;
-        if (have)
-            for (uint32_t f = 0; f < n2; ++f) {
-                const Cand o = L.cand[w2][f];
-                rank += cand_less(o.s, o.key, me.s, me.key) ? 1u : 0u;
-            }
-    }
-    if (slot < P.slot_cap) {
-        Cand *out = P.slots + ((size_t)cohort * P.slot_cap + slot) * kTop;
-        if (have && rank < (uint32_t)kTop)
-            out[rank] = me;
-        if (threadIdx.x < (uint32_t)kTop && threadIdx.x >= total) {
-            Cand e;
-            e.s = INFINITY;
-            e.pad = 0;
-            e.key = kEmptyKey;
-            out[threadIdx.x] = e;
-        }
-    } else if (threadIdx.x == 0) {
-        atomicExch(P.overflow, 1u);
-    }
-    __syncthreads();
-    cnt = 0;
-    ts = INFINITY;
-    tk = kEmptyKey;
-}
-
+// Wavefronts are independent workers: each pulls its own items (one i of a
+// (k-block, j-group) row) from the queue, prefetching the next index while
+// it counts.  The workgroup only shares the LDS table.
 template <int JP, int NW, bool TLDS, bool DUMP>
-// At least 4 wavefronts per SIMD (<= 128 VGPRs) to hide load latency.
 __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -362,17 +258,18 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = rfl(tid >> 6);
-    Cand *buf = L.cand[wave];
-    Cand *keep = buf + kWaveCap;
+    topk::WaveTop W;
+    W.buf = L.cand[wave];
+    W.keep = W.buf + kWaveCap;
+    W.reset();
 
-    if (TLDS)
+    if (TLDS) {
         for (uint32_t t = tid; t < P.table_len; t += NW * 64)
             Tl[t] = P.table[t];
+        __syncthreads();
+    }
 
-    int cnt = 0;
-    float ts = INFINITY;
-    uint64_t tk = kEmptyKey;
-    uint32_t cur = 0xFFFFFFFFu;  // cohort whose candidates the buffers hold
+    uint32_t cur = 0xFFFFFFFFu;  // cohort whose candidates W holds
     uint32_t sg = 0;             // current segment
     uint32_t r = 0;              // current row of that segment
     bool fresh = true;           // r must be reset to the segment's first row
@@ -380,12 +277,18 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
     const uint32_t n_segs = as_const(P.hdr)->n_segments;
     const uint32_t n_items = as_const(P.hdr)->n_items;
 
+    uint32_t q = 0;
+    if (lane == 0)
+        q = atomicAdd(&P.counters[0], 1u);
+    uint32_t next = rfl(q);
     for (;;) {
-        if (tid == 0)
-            L.misc[0] = atomicAdd(&P.counters[0], 1u);
-        __syncthreads();
-        const uint32_t item = rfl(L.misc[0]);
-        __syncthreads();
+        const uint32_t item = next;
+        if (item < n_items) {  // prefetch the following item while this one runs
+            uint32_t q2 = 0;
+            if (lane == 0)
+                q2 = atomicAdd(&P.counters[0], 1u);
+            next = rfl(q2);
+        }
         // Items are pulled in increasing order, so segment and row only move
         // forward: both are found by short forward walks.
         uint32_t cohort = 0xFFFFFFFFu;
@@ -409,18 +312,17 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
         }
         if (cohort != cur) {
             if (!DUMP && cur != 0xFFFFFFFFu)
-                wg_flush<NW>(P, L, cur, cnt, ts, tk, wave, lane);
+                topk::wave_flush(W, P, cur, lane);
             cur = cohort;
         }
         if (item >= n_items)
             break;
+        if (!DUMP)
+            W.gthr = topk::load_thr(P.thr, cohort);
 
         const uint32_t kb = rows[r].kb;
         const uint32_t jg = rows[r].jg;
-        const uint32_t icount = rows[r].icount;
-        const uint32_t i = (u - rows[r].item_start) * NW + wave;
-        if (i >= icount)
-            continue;
+        const uint32_t i = u - rows[r].item_start;  // one i per item
 
         const EPI_CONST CohortDev *C = as_const(P.cohorts) + cohort;
         const uint32_t M = C->M, Mp = C->Mp;
@@ -429,14 +331,14 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
 
         uint32_t acc0[JP][8], acc1[JP][8];
         {
-            const uint32_t W = C->W[0];
+            const uint32_t Wd = C->W[0];
             const uint4 *S = C->sm[0];
-            count8<JP>(S + (size_t)i * W, S + (size_t)j0 * W, W, C->wm[0] + k, Mp, acc0);
+            count8<JP>(S + (size_t)i * Wd, S + (size_t)j0 * Wd, Wd, C->wm[0] + k, Mp, acc0);
         }
         {
-            const uint32_t W = C->W[1];
+            const uint32_t Wd = C->W[1];
             const uint4 *S = C->sm[1];
-            count8<JP>(S + (size_t)i * W, S + (size_t)j0 * W, W, C->wm[1] + k, Mp, acc1);
+            count8<JP>(S + (size_t)i * Wd, S + (size_t)j0 * Wd, Wd, C->wm[1] + k, Mp, acc1);
         }
 
         const uint32_t n0 = C->n[0], n1 = C->n[1];
@@ -496,7 +398,7 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
                 }
             }
             if (!DUMP)
-                wave_offer(valid, s, make_key(i, j, k), buf, keep, cnt, ts, tk, lane);
+                topk::wave_offer(W, valid, s, make_key(i, j, k), lane);
         }
     }
 }
@@ -509,6 +411,8 @@ size_t lds_total(bool table_lds, uint32_t table_len)
 
 size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len)
 {
+    if (cfg.variant == 2)
+        return mfma_lds_bytes(table_lds, table_len);
     switch (cfg.waves) {
     case 4: return lds_total<4>(table_lds, table_len);
     case 8: return lds_total<8>(table_lds, table_len);
@@ -540,7 +444,7 @@ static ScanFn pick2(int nw, bool tl, bool dump)
 
 static ScanFn pick(const ScanConfig &cfg, bool tl, bool dump)
 {
-    if (cfg.words != 1)
+    if (cfg.words != 1 || cfg.variant != 1)
         return nullptr;
     switch (cfg.snps_j) {
     case 1: return pick2<1>(cfg.waves, tl, dump);
@@ -551,11 +455,18 @@ static ScanFn pick(const ScanConfig &cfg, bool tl, bool dump)
     }
 }
 
-bool scan_config_supported(const ScanConfig &cfg) { return pick(cfg, false, false) != nullptr; }
+bool scan_config_supported(const ScanConfig &cfg)
+{
+    if (cfg.variant == 2)
+        return cfg.snps_j == 64 && cfg.waves == 4 && cfg.words == 1;
+    return pick(cfg, false, false) != nullptr;
+}
 
 hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes,
                           int *blocks_per_cu)
 {
+    if (cfg.variant == 2)
+        return mfma_occupancy(table_lds, dump, lds_bytes, blocks_per_cu);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;
@@ -565,6 +476,8 @@ hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump, size
 
 hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes)
 {
+    if (cfg.variant == 2)
+        return mfma_allow_lds(table_lds, dump, lds_bytes);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;
@@ -575,6 +488,8 @@ hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size
 hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump, const ScanParams &p,
                        uint32_t grid, hipStream_t s)
 {
+    if (cfg.variant == 2)
+        return launch_mfma_scan(table_lds, dump, p, grid, s);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;
@@ -610,20 +525,22 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s)
 }
 
 // One workgroup per cohort: start from the cohort's running list, fold in
-// its slots 15 at a time (rank of each entry among 1024), write the list back.
+// the entries the wavefronts appended (960 at a time, rank of each entry
+// among 1024), write the list back.  Entries not better than the current
+// 64th best are skipped without ranking.
 __global__ void __launch_bounds__(1024)
 merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
-             const uint32_t *__restrict__ counters, uint32_t slot_cap, Cand *const *lists)
+             const uint32_t *__restrict__ counters, uint32_t cap, Cand *const *lists)
 {
     __shared__ Cand ent[1024];
     const uint32_t cohort = blockIdx.x;
     if (cohort >= hdr->n_cohorts)
         return;
     const uint32_t t = threadIdx.x;
-    uint32_t nslots = counters[1 + cohort];
-    if (nslots > slot_cap)
-        nslots = slot_cap;
-    const Cand *src = slots + (size_t)cohort * slot_cap * kTop;
+    uint32_t n = counters[1 + cohort];
+    if (n > cap)
+        n = cap;
+    const Cand *src = slots + (size_t)cohort * cap;
     Cand *list = lists[cohort];
     Cand empty;
     empty.s = INFINITY;
@@ -631,11 +548,11 @@ merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
     empty.key = kEmptyKey;
     if (t < (uint32_t)kTop)
         ent[t] = list[t];
-    constexpr uint32_t per_pass = 1024 / kTop - 1;
-    for (uint32_t base = 0; base < nslots; base += per_pass) {
+    constexpr uint32_t per_pass = 1024 - kTop;
+    for (uint32_t base = 0; base < n; base += per_pass) {
         if (t >= (uint32_t)kTop) {
-            const uint32_t sl = base + (t - kTop) / kTop;
-            ent[t] = sl < nslots ? src[(size_t)sl * kTop + (t % kTop)] : empty;
+            const uint32_t e = base + (t - kTop);
+            ent[t] = e < n ? src[e] : empty;
         }
         __syncthreads();
         const Cand me = ent[t];
@@ -658,6 +575,27 @@ merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
     }
     if (t < (uint32_t)kTop)
         list[t] = ent[t];
+}
+
+// Per-launch cohort thresholds from the running lists (64th best score;
+// +inf while a list holds fewer than 64 candidates).
+__global__ void init_thr_kernel(const RoundHdr *__restrict__ hdr, Cand *const *lists,
+                                uint32_t *thr, uint32_t max_cohorts)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= max_cohorts)
+        return;
+    thr[c] = c < hdr->n_cohorts ? __float_as_uint(lists[c][kTop - 1].s) : 0x7F800000u;
+}
+
+hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *thr,
+                           uint32_t max_cohorts, hipStream_t s)
+{
+    if (max_cohorts == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(init_thr_kernel, dim3((max_cohorts + 63) / 64), dim3(64), 0, s, hdr, lists,
+                       thr, max_cohorts);
+    return hipGetLastError();
 }
 
 hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,

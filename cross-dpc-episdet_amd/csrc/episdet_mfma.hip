@@ -1,0 +1,423 @@
+// episdet_mfma.hip -- int8 matrix-core formulation of the 8 popcounted
+// cells (the alternative to the VALU AND+popcount scan kernel).
+//
+// For one class and a fixed SNP i, the counts the VALU kernel popcounts,
+//   t[a][b][c](i,j,k) = sum_s [g_i(s)=a] [g_j(s)=b] [g_k(s)=c],  a,b,c in {0,1},
+// are a matrix product over samples s:
+//   A[(j,a,b)][s] = [g_i(s)=a] & [g_j(s)=b]      (0/1 bytes)
+//   B[s][(k,c)]   = [g_k(s)=c]                    (0/1 bytes)
+//   C = A B       with exact int32 accumulation,
+// computed with v_mfma_i32_32x32x32_i8.  The 0/1 bytes come from a per-
+// cohort expansion of the bit planes (E[c][snp][plane][sample]).
+//
+// Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
+// 4 wavefronts.  Per stage of 64 samples the workgroup stages the j-block
+// and k-block bytes (and SNP i's) in LDS; wavefront w computes the 64 rows
+// (16 j x 4 (a,b)) of its quarter against all 128 columns (64 k x 2 c):
+// 2 x 4 MFMA tiles of 32 x 32 per class, 256 int32 accumulators.  The
+// epilogue is the VALU kernel's: exact 27-cell reconstruction from the
+// margins, K2 in the reference order, per-wavefront top-64.
+#include <math.h>
+
+#include "episdet_cells.h"
+#include "episdet_internal.h"
+#include "episdet_topk.h"
+
+#define EPI_CONST __attribute__((address_space(4)))
+
+namespace epi {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+namespace mf {
+
+template <typename T>
+__device__ __forceinline__ const EPI_CONST T *as_const(const T *p)
+{
+    return (const EPI_CONST T *)p;
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint4 ldc4(const uint4 *p)
+{
+    const EPI_CONST uint32_t *q = (const EPI_CONST uint32_t *)p;
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ uint2 ldc2(const uint2 *p)
+{
+    const EPI_CONST uint32_t *q = (const EPI_CONST uint32_t *)p;
+    return make_uint2(q[0], q[1]);
+}
+
+}  // namespace mf
+
+// ------------------------------------------------------------------------
+// bit planes -> 0/1 bytes: E[c] row (snp, plane) holds EL bytes (64 W_c
+// rounded up to a whole 128-byte stage), byte t = bit t of the plane
+// (sample t of the class); padding bytes are 0.  One thread per (snp,
+// 64-bit word slot, plane) writes 64 bytes; slots w >= W write zeros.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+expand_kernel(const uint4 *__restrict__ sm, uint32_t Mp, uint32_t W, uint32_t EL,
+              uint8_t *__restrict__ E)
+{
+    const size_t t = blockIdx.x * (size_t)256 + threadIdx.x;
+    const uint32_t WS = EL / 64;  // word slots per row
+    const size_t total = (size_t)Mp * WS * 2;
+    if (t >= total)
+        return;
+    const uint32_t plane = (uint32_t)(t & 1);
+    const size_t sw = t >> 1;  // snp * WS + w
+    const uint32_t w = (uint32_t)(sw % WS);
+    const size_t snp = sw / WS;
+    const uint4 v = w < W ? sm[snp * W + w] : make_uint4(0, 0, 0, 0);
+    const uint32_t lo = plane ? v.z : v.x;
+    const uint32_t hi = plane ? v.w : v.y;
+    uint32_t out[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t word = q < 8 ? lo : hi;
+        const uint32_t nib = (word >> (4 * (q & 7))) & 0xFu;
+        // spread 4 bits to the low bit of 4 bytes
+        out[q] = (nib * 0x00204081u) & 0x01010101u;
+    }
+    uint4 *dst = (uint4 *)(E + (snp * 2 + plane) * (size_t)EL + (size_t)w * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        dst[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+}
+
+hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, uint8_t *E,
+                         hipStream_t s)
+{
+    const size_t total = (size_t)Mp * (EL / 64) * 2;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(expand_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, sm,
+                       Mp, W, EL, E);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// the MFMA scan kernel
+// ------------------------------------------------------------------------
+constexpr int kMfWaves = 4;
+constexpr int kStage = 128;           // samples per LDS stage (four 32-sample MFMA steps)
+constexpr int kRowPitch = kStage + 16;  // LDS bytes per staged (snp, plane) row, padded
+constexpr int kStageLoads = 2 * 64 * 2 * kStage / 16 / (kMfWaves * 64);  // uint4 per thread
+constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
+
+struct MfLds {
+    Cand cand[kMfWaves][kWaveCap + kTop];
+    uint32_t misc[4];
+    uint8_t jrows[kStageRows * kRowPitch];  // j-block bytes of the stage
+    uint8_t krows[kStageRows * kRowPitch];  // k-block bytes of the stage
+    uint8_t irow[2 * kStage];               // SNP i, both planes
+};
+
+__host__ __device__ constexpr size_t mf_lds_fixed() { return (sizeof(MfLds) + 15) & ~(size_t)15; }
+
+// 16 bytes of LDS as an MFMA operand
+__device__ __forceinline__ v4i lds16(const uint8_t *p) { return *(const v4i *)p; }
+
+// Loads one stage into registers: thread t takes uint4 number idx = t + 256 q,
+// block (j/k) = idx / 1024, row (snp_local, plane) = (idx / 8) % 128,
+// 16-byte chunk = idx % 8; threads 0..15 also take SNP i's two rows.
+__device__ __forceinline__ void mf_load_stage(v4i (&pre)[kStageLoads], v4i &pre_i,
+                                              const uint8_t *E, uint32_t Lc, uint32_t s0,
+                                              uint32_t J0, uint32_t K0, uint32_t i, uint32_t tid)
+{
+#pragma unroll
+    for (int q = 0; q < kStageLoads; ++q) {
+        const uint32_t idx = tid + 256u * q;
+        const uint32_t blk = idx >> 10;
+        const uint32_t rowi = (idx >> 3) & 127u;
+        const uint32_t snp = (blk ? K0 : J0) + (rowi >> 1);
+        pre[q] = *(const v4i *)(E + ((size_t)snp * 2 + (rowi & 1)) * Lc + s0 + 16 * (idx & 7u));
+    }
+    if (tid < 16)
+        pre_i = *(const v4i *)(E + ((size_t)i * 2 + (tid >> 3)) * Lc + s0 + 16 * (tid & 7u));
+}
+
+template <bool TLDS, bool DUMP>
+__global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanParams P)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    MfLds &L = *reinterpret_cast<MfLds *>(lds_raw);
+    float *Tl = reinterpret_cast<float *>(lds_raw + mf_lds_fixed());
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = mf::rfl(tid >> 6);
+    topk::WaveTop Wt;
+    Wt.buf = L.cand[wave];
+    Wt.keep = Wt.buf + kWaveCap;
+    Wt.reset();
+
+    if (TLDS)
+        for (uint32_t t = tid; t < P.table_len; t += kMfWaves * 64)
+            Tl[t] = P.table[t];
+
+    uint32_t cur = 0xFFFFFFFFu;
+    uint32_t sg = 0, r = 0;
+    bool fresh = true;
+    const EPI_CONST Segment *segs = mf::as_const(P.segs);
+    const uint32_t n_segs = mf::as_const(P.hdr)->n_segments;
+    const uint32_t n_items = mf::as_const(P.hdr)->n_items;
+
+    // lane roles in the MFMA tiles
+    const uint32_t half = lane >> 5;   // which 16 bytes of a 32-sample step
+    const uint32_t l32 = lane & 31u;   // A row / B column within a tile
+
+    for (;;) {
+        if (tid == 0)
+            L.misc[0] = atomicAdd(&P.counters[0], 1u);
+        __syncthreads();
+        const uint32_t item = mf::rfl(L.misc[0]);
+        __syncthreads();
+        uint32_t cohort = 0xFFFFFFFFu, u = 0;
+        const EPI_CONST Row *rows = nullptr;
+        if (item < n_items) {
+            while (sg + 1 < n_segs && segs[sg + 1].item_base <= item) {
+                ++sg;
+                fresh = true;
+            }
+            rows = (const EPI_CONST Row *)segs[sg].rows;
+            if (fresh) {
+                r = segs[sg].row_begin;
+                fresh = false;
+            }
+            u = item - segs[sg].item_base + segs[sg].item_first;
+            const uint32_t row_end = segs[sg].row_end;
+            while (r + 1 < row_end && rows[r + 1].item_start <= u)
+                ++r;
+            cohort = segs[sg].cohort;
+        }
+        if (cohort != cur) {
+            if (!DUMP && cur != 0xFFFFFFFFu)
+                topk::wave_flush(Wt, P, cur, lane);
+            cur = cohort;
+        }
+        if (item >= n_items)
+            break;
+        if (!DUMP)
+            Wt.gthr = topk::load_thr(P.thr, cohort);
+
+        const uint32_t K0 = rows[r].kb * 64;
+        const uint32_t J0 = rows[r].jg * 64;
+        const uint32_t i = u - rows[r].item_start;  // one i per item
+        const EPI_CONST CohortDev *C = mf::as_const(P.cohorts) + cohort;
+        const uint32_t M = C->M, Mp = C->Mp;
+
+        v16i acc[2][2][4];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int fa = 0; fa < 2; ++fa)
+#pragma unroll
+                for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                        acc[c][fa][fb][e] = 0;
+
+        // Stage loop: the next stage's bytes are loaded into registers while
+        // the current stage (in LDS) feeds the MFMAs.
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint8_t *E = C->E[c];
+            const uint32_t Lc = C->EL[c];  // bytes per (snp, plane) row, a multiple of kStage
+            if (Lc == 0)
+                continue;
+            // thread t stages uint4 number idx = t + 256 q of the stage:
+            // block (j/k) = idx / 1024, row (snp_local, plane) = (idx / 8) % 128,
+            // 16-byte chunk = idx % 8
+            v4i pre[kStageLoads];
+            v4i pre_i = {0, 0, 0, 0};
+            mf_load_stage(pre, pre_i, E, Lc, 0u, J0, K0, i, tid);
+            for (uint32_t s0 = 0; s0 < Lc; s0 += kStage) {
+                __syncthreads();  // the previous stage is no longer read
+#pragma unroll
+                for (int q = 0; q < kStageLoads; ++q) {
+                    const uint32_t idx = tid + 256u * q;
+                    const uint32_t blk = idx >> 10;
+                    const uint32_t rowi = (idx >> 3) & 127u;
+                    const uint32_t chunk = idx & 7u;
+                    *(v4i *)((blk ? L.krows : L.jrows) + rowi * kRowPitch + 16 * chunk) = pre[q];
+                }
+                if (tid < 16)
+                    *(v4i *)(L.irow + (tid >> 3) * kStage + 16 * (tid & 7u)) = pre_i;
+                if (s0 + kStage < Lc)
+                    mf_load_stage(pre, pre_i, E, Lc, s0 + kStage, J0, K0, i, tid);
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < kStage / 32; ++kk) {
+                    const uint32_t off = kk * 32 + 16 * half;
+                    // A: row l32 of tile fa = (j_local, a, b) with j_local = 16w + 8fa + l32/4
+                    v4i A[2];
+#pragma unroll
+                    for (int fa = 0; fa < 2; ++fa) {
+                        const uint32_t jl = 16 * wave + 8 * fa + (l32 >> 2);
+                        const uint32_t a = (l32 >> 1) & 1u, b = l32 & 1u;
+                        const v4i ej = lds16(L.jrows + (jl * 2 + b) * kRowPitch + off);
+                        const v4i ei = lds16(L.irow + a * kStage + off);
+                        A[fa] = ej & ei;
+                    }
+#pragma unroll
+                    for (int fb = 0; fb < 4; ++fb) {
+                        // B: column l32 of tile fb = (k_local, c) with k_local = 16fb + l32/2
+                        const uint32_t kl = 16 * fb + (l32 >> 1), cc = l32 & 1u;
+                        const v4i Bv = lds16(L.krows + (kl * 2 + cc) * kRowPitch + off);
+#pragma unroll
+                        for (int fa = 0; fa < 2; ++fa)
+                            acc[c][fa][fb] =
+                                __builtin_amdgcn_mfma_i32_32x32x32_i8(A[fa], Bv, acc[c][fa][fb], 0, 0, 0);
+                    }
+                }
+            }
+        }
+
+        // ---- epilogue: lane holds column (k, cc) and rows (j, ab) with
+        // row = 8 (e/4) + 4 half + e%4, i.e. j_local = 16w + 8fa + 2 (e/4) + half,
+        // ab = e%4, for accumulator element e.  The partner lane (lane ^ 1)
+        // holds the other cc; even lanes score j-slots 0,1, odd lanes 2,3.
+        // One tile at a time (not unrolled) keeps the scoring code single.
+        const uint32_t n0 = C->n[0], n1 = C->n[1];
+        const uint32_t tmax = P.table_len - 1;
+        const bool odd = (lane & 1u) != 0;
+#pragma unroll 1
+        for (int t = 0; t < 8; ++t) {
+            int v[2][16];
+            switch (t) {
+#define EPI_MF_COPY(T, FA, FB)                      \
+    case T:                                          \
+        for (int e = 0; e < 16; ++e) {               \
+            v[0][e] = acc[0][FA][FB][e];             \
+            v[1][e] = acc[1][FA][FB][e];             \
+        }                                            \
+        break;
+                EPI_MF_COPY(0, 0, 0)
+                EPI_MF_COPY(1, 0, 1)
+                EPI_MF_COPY(2, 0, 2)
+                EPI_MF_COPY(3, 0, 3)
+                EPI_MF_COPY(4, 1, 0)
+                EPI_MF_COPY(5, 1, 1)
+                EPI_MF_COPY(6, 1, 2)
+                EPI_MF_COPY(7, 1, 3)
+#undef EPI_MF_COPY
+            }
+            const uint32_t fa = (uint32_t)t >> 2, fb = (uint32_t)t & 3u;
+            const uint32_t k = K0 + 16 * fb + (l32 >> 1);
+            int w[2][16];  // the partner lane's values
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    w[c][e] = __shfl_xor(v[c][e], 1);
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const uint32_t slot = (odd ? 2u : 0u) + (uint32_t)sl;  // e / 4
+                const uint32_t j = J0 + 16 * wave + 8 * fa + 2 * slot + half;
+                const bool valid = (i < j) && (j < k) && (k < M);
+                float s = INFINITY;
+                if (valid) {
+                    uint32_t x0[27], x1[27];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        uint32_t t8[8];
+#pragma unroll
+                        for (int ab = 0; ab < 4; ++ab) {
+                            // element of this lane's j-slot: even lanes slots 0,1, odd 2,3
+                            const uint32_t mine = (uint32_t)(odd ? v[c][8 + 4 * sl + ab] : v[c][4 * sl + ab]);
+                            const uint32_t theirs = (uint32_t)(odd ? w[c][8 + 4 * sl + ab] : w[c][4 * sl + ab]);
+                            t8[2 * ab + 0] = odd ? theirs : mine;  // cc = 0 lives in the even lane
+                            t8[2 * ab + 1] = odd ? mine : theirs;
+                        }
+                        const uint4 ij = C->pair[c][(size_t)i * Mp + j];
+                        const uint4 ik = C->pair[c][(size_t)i * Mp + k];
+                        const uint4 jk = C->pair[c][(size_t)j * Mp + k];
+                        const uint2 si = mf::ldc2(C->single[c] + i);
+                        const uint2 sj = C->single[c][j];
+                        const uint2 sk = C->single[c][k];
+                        const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
+                        const uint32_t aik[4] = {ik.x, ik.y, ik.z, ik.w};
+                        const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
+                        const uint32_t asi[2] = {si.x, si.y};
+                        const uint32_t asj[2] = {sj.x, sj.y};
+                        const uint32_t ask[2] = {sk.x, sk.y};
+                        derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n1 : n0, c ? x1 : x0);
+                    }
+                    float sc = 0.0f;
+#pragma unroll
+                    for (int m = 0; m < 27; ++m) {
+                        const uint32_t ia = min(x0[m] + x1[m] + 1, tmax);
+                        const uint32_t ib = min(x0[m], tmax);
+                        const uint32_t id = min(x1[m], tmax);
+                        const float av = TLDS ? Tl[ia] : P.table[ia];
+                        const float bv = TLDS ? Tl[ib] : P.table[ib];
+                        const float dv = TLDS ? Tl[id] : P.table[id];
+                        sc += (av - bv) - dv;
+                        if (!TLDS && (m % 3) == 2)
+                            __builtin_amdgcn_sched_barrier(0);
+                    }
+                    s = fabsf(sc);
+                    if (DUMP) {
+                        const uint64_t MM = M;
+                        auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; };
+                        auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
+                        const uint64_t rk = c3(MM) - c3(MM - i) + c2(MM - i - 1) - c2(MM - j) +
+                                            (uint64_t)(k - j - 1);
+                        P.dump_scores[rk] = s;
+                        if (P.dump_counts) {
+#pragma unroll
+                            for (int m = 0; m < 27; ++m) {
+                                P.dump_counts[rk * 54 + m] = x0[m];
+                                P.dump_counts[rk * 54 + 27 + m] = x1[m];
+                            }
+                        }
+                    }
+                }
+                if (!DUMP)
+                    topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
+            }
+        }
+    }
+}
+
+size_t mfma_lds_bytes(bool table_lds, uint32_t table_len)
+{
+    return mf_lds_fixed() + (table_lds ? (size_t)table_len * 4 : 0);
+}
+
+typedef void (*MfFn)(const ScanParams);
+
+static MfFn mf_pick(bool tl, bool dump)
+{
+    if (tl)
+        return dump ? mfma_scan_kernel<true, true> : mfma_scan_kernel<true, false>;
+    return dump ? mfma_scan_kernel<false, true> : mfma_scan_kernel<false, false>;
+}
+
+hipError_t mfma_occupancy(bool table_lds, bool dump, size_t lds_bytes, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        (const void *)mf_pick(table_lds, dump),
+                                                        kMfWaves * 64, lds_bytes);
+}
+
+hipError_t mfma_allow_lds(bool table_lds, bool dump, size_t lds_bytes)
+{
+    return hipFuncSetAttribute((const void *)mf_pick(table_lds, dump),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+hipError_t launch_mfma_scan(bool table_lds, bool dump, const ScanParams &p, uint32_t grid,
+                            hipStream_t s)
+{
+    const size_t lds = mfma_lds_bytes(table_lds, p.table_len);
+    hipLaunchKernelGGL(mf_pick(table_lds, dump), dim3(grid), dim3(kMfWaves * 64), lds, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace epi

@@ -1,0 +1,162 @@
+// episdet_topk.h -- per-wavefront top-64 candidate selection, shared by the
+// scan kernels (device code).
+//
+// Replaces the reference's per-slot running minimum and host argmin
+// (GPU/v4.cpp:586-608).  Candidates are ordered by (FP32 score, key), key =
+// i << 42 | j << 21 | k, so ties go to the lexicographically first triplet.
+//   wave_offer   every lane offers one triplet; a ballot keeps those better
+//                than the wave's current 64th best and than the cohort's
+//                global threshold, appended to the wave's LDS buffer;
+//   wave_compact ranks the buffer (keys are unique) and keeps the best 64,
+//                sorted;
+//   wave_flush   when a wave leaves a cohort: appends its entries that do not
+//                exceed the cohort's global threshold to the cohort's
+//                candidate buffer (one atomic per wave) and lowers the
+//                threshold to the wave's 64th best score.
+// The global threshold is only ever the 64th best score of some set of 64
+// real candidates, so it bounds the cohort's true 64th best from above and
+// nothing of the true top 64 is dropped.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "episdet_internal.h"
+
+namespace epi {
+namespace topk {
+
+__device__ __forceinline__ bool less(float s, uint64_t k, float ts, uint64_t tk)
+{
+    return s < ts || (s == ts && k < tk);
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Per-wavefront state; cnt, ts, tk are wave-uniform.
+struct WaveTop {
+    Cand *buf;   // kWaveCap entries
+    Cand *keep;  // kTop entries
+    int cnt;
+    float ts;    // the wave's 64th best (score, key) once it has 64
+    uint64_t tk;
+    float gthr;  // cohort-wide score threshold (may be stale, never too low)
+
+    __device__ void reset()
+    {
+        cnt = 0;
+        ts = INFINITY;
+        tk = kEmptyKey;
+        gthr = INFINITY;
+    }
+};
+
+struct Compacted {
+    int cnt;
+    float ts;
+    uint64_t tk;
+};
+
+// Kept out of line (it runs rarely); state goes in and out by value so the
+// caller's registers are not forced into scratch memory.
+__device__ __noinline__ Compacted compact_buf(Cand *buf, Cand *keep, int n, float ts, uint64_t tk,
+                                              uint32_t lane)
+{
+    wave_sync();
+    for (int e = (int)lane; e < n; e += kLanes) {
+        const Cand me = buf[e];
+        int rank = 0;
+        for (int f = 0; f < n; ++f) {
+            const Cand o = buf[f];
+            rank += less(o.s, o.key, me.s, me.key) ? 1 : 0;
+        }
+        if (rank < kTop)
+            keep[rank] = me;
+    }
+    wave_sync();
+    const int m = n < kTop ? n : kTop;
+    if ((int)lane < m)
+        buf[lane] = keep[lane];
+    Compacted r;
+    r.cnt = m;
+    r.ts = ts;
+    r.tk = tk;
+    if (m == kTop) {
+        r.ts = keep[kTop - 1].s;
+        r.tk = keep[kTop - 1].key;
+    }
+    wave_sync();
+    return r;
+}
+
+__device__ __forceinline__ void wave_compact(WaveTop &W, uint32_t lane)
+{
+    const Compacted r = compact_buf(W.buf, W.keep, W.cnt, W.ts, W.tk, lane);
+    W.cnt = __builtin_amdgcn_readfirstlane(r.cnt);
+    W.ts = r.ts;
+    W.tk = r.tk;
+}
+
+__device__ __forceinline__ void wave_offer(WaveTop &W, bool valid, float s, uint64_t key,
+                                           uint32_t lane)
+{
+    const bool pass = valid && s <= W.gthr && less(s, key, W.ts, W.tk);
+    const uint64_t mask = __ballot(pass);
+    if (mask == 0)
+        return;
+    const int nn = __popcll(mask);
+    if (W.cnt + nn > kWaveCap)
+        wave_compact(W, lane);
+    if (pass) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        Cand c;
+        c.s = s;
+        c.pad = 0;
+        c.key = key;
+        W.buf[W.cnt + below] = c;
+    }
+    W.cnt += nn;
+}
+
+// Reads the cohort's global threshold (scores are >= 0, so their bit
+// patterns order like the values and the threshold is kept as uint32).
+__device__ __forceinline__ float load_thr(const uint32_t *thr, uint32_t cohort)
+{
+    return __uint_as_float(__atomic_load_n(thr + cohort, __ATOMIC_RELAXED));
+}
+
+__device__ void wave_flush(WaveTop &W, const ScanParams &P, uint32_t cohort, uint32_t lane)
+{
+    if (W.cnt > 0) {
+        wave_compact(W, lane);  // sorted, at most kTop entries
+        const float g = load_thr(P.thr, cohort);
+        const bool pass = (int)lane < W.cnt && W.buf[lane].s <= g;
+        const uint64_t mask = __ballot(pass);
+        const uint32_t nn = (uint32_t)__popcll(mask);
+        if (nn) {
+            uint32_t base = 0;
+            if (lane == 0)
+                base = atomicAdd(&P.counters[1 + cohort], nn);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base + nn <= P.slot_cap) {
+                if (pass) {
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                    P.slots[(size_t)cohort * P.slot_cap + base + below] = W.buf[lane];
+                }
+            } else if (lane == 0) {
+                atomicExch(P.overflow, 1u);
+            }
+        }
+        if (W.cnt == kTop && lane == 0)
+            atomicMin(P.thr + cohort, __float_as_uint(W.buf[kTop - 1].s));
+    }
+    W.reset();
+}
+
+}  // namespace topk
+}  // namespace epi

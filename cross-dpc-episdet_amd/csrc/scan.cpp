@@ -90,6 +90,14 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
 {
     ScanConfig cfg;
+    if (tile && tile->variant == 2) {
+        cfg.variant = 2;  // int8 MFMA: 64-SNP j- and k-blocks, 4 wavefronts
+        cfg.snps_j = 64;
+        cfg.waves = 4;
+        cfg.words = 1;
+        return cfg;
+    }
+    cfg.variant = 1;
     cfg.words = 1;
     cfg.snps_j = 4;
     cfg.waves = 8;
@@ -103,6 +111,9 @@ ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_s
                 cfg.snps_j = 2;
         }
     }
+    if (cfg.variant == 2 && !(tile && tile->variant == 1))
+        return cfg;
+    cfg.variant = 1;
     if (tile) {
         if (tile->snps_j)
             cfg.snps_j = (int)tile->snps_j;
@@ -131,6 +142,9 @@ int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool du
     if (per_cu <= 0)
         return EPI_EDEVICE;
     plan.grid = (uint32_t)per_cu * (uint32_t)ctx->cu_count;
+    // every wavefront flushes each cohort at most once per launch, at most
+    // kTop candidates each
+    plan.cap = plan.grid * (uint32_t)cfg.waves * kTop;
     return EPI_OK;
 }
 
@@ -140,14 +154,35 @@ int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg)
     job.rows.clear();
     job.items = 0;
     job.next_row = 0;
-    build_rows(0, co->M, co->Mp, cfg.snps_j, cfg.waves, job.rows, job.items);
+    // one i per queue item: a wavefront (VALU kernel) or a workgroup (MFMA
+    // kernel) takes one item at a time
+    build_rows(0, co->M, co->Mp, cfg.snps_j, 1, job.rows, job.items);
     if (job.items >= 0x7FFFFFFFull)
         return EPI_EINVAL;
-    job.nw = (uint32_t)cfg.waves;
+    job.nw = 1;
+    if (cfg.variant == 2)
+        EPI_TRY(ensure_expanded(ctx, co));
     EPI_TRY(job.list.reserve(sizeof(Cand) * kTop));
     EPI_HIP(launch_fill_empty((Cand *)job.list.ptr, kTop, ctx->stream));
     EPI_TRY(job.drows.reserve(sizeof(Row) * std::max<size_t>(job.rows.size(), 1)));
     EPI_TRY(stage_upload(ctx, job.drows.ptr, job.rows.data(), sizeof(Row) * job.rows.size()));
+    return EPI_OK;
+}
+
+int ensure_expanded(epi_ctx *ctx, epi_cohort *co)
+{
+    for (int c = 0; c < 2; ++c) {
+        if (co->E[c].ptr)
+            continue;
+        const uint32_t EL = (co->W[c] + 1) / 2 * 128;  // whole 128-sample stages
+        const size_t bytes = (size_t)co->Mp * 2 * std::max<uint32_t>(EL, 128);
+        EPI_TRY(co->E[c].reserve(bytes));
+        co->EL[c] = EL;
+        if (EL)
+            EPI_HIP(launch_expand((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], EL,
+                                  (uint8_t *)co->E[c].ptr, ctx->stream));
+        co->bytes += bytes;
+    }
     return EPI_OK;
 }
 
@@ -237,8 +272,9 @@ int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint3
 int reserve_round(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, size_t bytes)
 {
     EPI_TRY(ctx->round.reserve(bytes));
-    EPI_TRY(ctx->counters.reserve(sizeof(uint32_t) * (2 + (size_t)n)));
-    EPI_TRY(ctx->slots.reserve(sizeof(Cand) * (size_t)n * plan.grid * kTop));
+    // [0] queue head, [1..n] appended counts, [1+n] overflow, [2+n..2+2n) thresholds
+    EPI_TRY(ctx->counters.reserve(sizeof(uint32_t) * (2 + 2 * (size_t)n)));
+    EPI_TRY(ctx->slots.reserve(sizeof(Cand) * (size_t)n * plan.cap));
     return EPI_OK;
 }
 
@@ -257,13 +293,15 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
     P.cohorts = (const CohortDev *)(base + L.descs);
     P.segs = (const Segment *)(base + L.segs);
     P.counters = counters;
+    P.thr = counters + 2 + n;
     P.table = (const float *)ctx->t32.ptr;
     P.table_len = plan.table_len;
-    P.slot_cap = plan.grid;
+    P.slot_cap = plan.cap;
     P.slots = (Cand *)ctx->slots.ptr;
     P.overflow = counters + 1 + n;
     P.dump_scores = dump_scores;
     P.dump_counts = dump_counts;
+    EPI_HIP(launch_init_thr(P.hdr, (Cand *const *)(base + L.lists), P.thr, n, ctx->stream));
     if (timed)
         EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (any_items)
@@ -271,7 +309,7 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
     if (timed)
         EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     if (!plan.dump)
-        EPI_HIP(launch_merge(P.hdr, P.slots, counters, plan.grid, (Cand *const *)(base + L.lists),
+        EPI_HIP(launch_merge(P.hdr, P.slots, counters, plan.cap, (Cand *const *)(base + L.lists),
                              n, ctx->stream));
     return EPI_OK;
 }
@@ -421,8 +459,8 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
     if (!result)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
-    if (tile && tile->variant > 1)
-        return EPI_EINVAL;  // only the VALU popcount path is built
+    if (tile && tile->variant > 2)
+        return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     EPI_TRY(ctx_ensure_tables(ctx, cohort->N));
@@ -450,7 +488,7 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     if (!results)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, cohorts, n));
-    if (tile && tile->variant > 1)
+    if (tile && tile->variant > 2)
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
@@ -480,7 +518,8 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     EPI_TRY(upload_rescore(ctx, jp.data(), n));
 
     std::vector<uint64_t> sig = {(uint64_t)n, (uint64_t)(items > 0), (uint64_t)plan.cfg.snps_j,
-                                 (uint64_t)plan.cfg.waves, (uint64_t)plan.grid, plan.table_len,
+                                 (uint64_t)plan.cfg.waves, (uint64_t)plan.grid, plan.cap,
+                                 (uint64_t)plan.cfg.variant, plan.table_len,
                                  (uint64_t)plan.table_lds, L.total,
                                  (uint64_t)(uintptr_t)ctx->round.ptr,
                                  (uint64_t)(uintptr_t)ctx->counters.ptr,
@@ -523,7 +562,7 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
     if (!scores)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
-    if (cohort->M > 512 || (tile && tile->variant > 1))
+    if (cohort->M > 512 || (tile && tile->variant > 2))
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));

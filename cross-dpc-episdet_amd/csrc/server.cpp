@@ -170,7 +170,7 @@ int epi_server::round(std::vector<Request *> &act)
     EPI_TRY(build_round(jp.data(), ranges.data(), n, cap, host, L, items));
     EPI_TRY(reserve_round(ctx, plan, cap, L.total));
     EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
-    std::vector<uint64_t> sig = {(uint64_t)cap, (uint64_t)plan.grid, L.total,
+    std::vector<uint64_t> sig = {(uint64_t)cap, (uint64_t)plan.grid, (uint64_t)plan.cap, L.total,
                                  (uint64_t)(uintptr_t)ctx->round.ptr,
                                  (uint64_t)(uintptr_t)ctx->counters.ptr,
                                  (uint64_t)(uintptr_t)ctx->slots.ptr,

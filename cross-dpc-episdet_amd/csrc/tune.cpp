@@ -32,10 +32,12 @@ int tune_load_file(epi_ctx *ctx, const std::string &path)
     while (std::fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n')
             continue;
-        unsigned ms = 0, sj = 0, w = 0;
-        if (std::sscanf(line, "%u %u %u", &ms, &sj, &w) == 3) {
+        unsigned ms = 0, sj = 0, w = 0, var = 1;
+        const int got = std::sscanf(line, "%u %u %u %u", &ms, &sj, &w, &var);
+        if (got >= 3) {
             TuneEntry e;
             e.max_samples = ms;
+            e.cfg.variant = got == 4 ? (int)var : 1;
             e.cfg.snps_j = (int)sj;
             e.cfg.waves = (int)w;
             e.cfg.words = 1;
@@ -86,8 +88,9 @@ extern "C" int epi_tune_set(epi_ctx *ctx, const uint32_t *max_samples, const epi
     for (uint32_t e = 0; e < n; ++e) {
         TuneEntry x;
         x.max_samples = max_samples[e];
-        x.cfg.snps_j = (int)tiles[e].snps_j;
-        x.cfg.waves = (int)tiles[e].waves;
+        x.cfg.variant = tiles[e].variant == 2 ? 2 : 1;
+        x.cfg.snps_j = x.cfg.variant == 2 ? 64 : (int)tiles[e].snps_j;
+        x.cfg.waves = x.cfg.variant == 2 ? 4 : (int)tiles[e].waves;
         x.cfg.words = 1;
         if (!scan_config_supported(x.cfg) || (e && max_samples[e] <= max_samples[e - 1]))
             return EPI_EINVAL;
@@ -105,7 +108,7 @@ extern "C" int epi_tune_get(epi_ctx *ctx, uint32_t n_samples, epi_tile *tile)
     std::lock_guard<std::mutex> lk(ctx->mu);
     const ScanConfig c = resolve_config(ctx, nullptr, n_samples);
     std::memset(tile, 0, sizeof(*tile));
-    tile->variant = 1;
+    tile->variant = (uint32_t)c.variant;
     tile->snps_j = (uint32_t)c.snps_j;
     tile->waves = (uint32_t)c.waves;
     tile->words = (uint32_t)c.words;
