@@ -41,6 +41,8 @@ extern "C" const char *epi_strerror(int code)
     case EPI_EAGAIN: return "not ready";
     case EPI_ENOENT: return "no such request";
     case EPI_EINTERNAL: return "internal consistency check failed";
+    case EPI_EIO: return "file cannot be opened, read or written";
+    case EPI_EFORMAT: return "malformed input file";
     default: return "unknown error";
     }
 }
@@ -228,47 +230,64 @@ bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps)
     return n_samples > 0 && n_snps >= 3 && n_snps <= (1u << 21) - 1;
 }
 
-// Device bytes a cohort of this shape holds (planes in two layouts, pair
-// and single margins, descriptor), the unit of server admission.
+// Device bytes a cohort of this shape may hold (planes in two layouts, pair
+// and single margins, descriptor, and the 0/1-byte expansion the MFMA scan
+// adds: 2 planes x Mp rows x (n_c rounded up to 128) bytes per class), the
+// unit of server admission.
 uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps)
 {
     const uint64_t Mp = (n_snps + 63) / 64 * 64;
     const uint64_t words = (n_samples + 63) / 64 + 2;  // both classes, rounded up
-    return 2 * words * Mp * 16 + 2 * Mp * Mp * 16 + 2 * Mp * 8 + sizeof(CohortDev) + 4096;
+    const uint64_t expanded = 2 * Mp * ((uint64_t)n_samples + 256);
+    return 2 * words * Mp * 16 + 2 * Mp * Mp * 16 + 2 * Mp * 8 + expanded + sizeof(CohortDev) +
+           4096;
 }
 
-// Caller holds ctx->mu.
-int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
-                         uint32_t n_samples, uint32_t n_snps, epi_cohort **out)
+// Caller holds ctx->mu.  Exactly one of geno (sample-major bytes) and bed
+// (PLINK 1 .bed body, unpacked on the device) is given.  With geno, any
+// phenotype other than 1 is a control (GPU/v4.cpp:141); with bed, samples
+// whose phenotype is neither 0 nor 1 are left out, as PLINK leaves out
+// samples with a missing phenotype.
+int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
+                         const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
+                         epi_cohort **out)
 {
+    // class split in sample order (GPU/v4.cpp:135-171): phenotype 1 = case
+    std::vector<uint32_t> order;
+    order.reserve(n_samples);
+    for (uint32_t s = 0; s < n_samples; ++s)
+        if (bed ? pheno[s] == 0 : pheno[s] != 1)
+            order.push_back(s);
+    const uint32_t n0 = (uint32_t)order.size();
+    for (uint32_t s = 0; s < n_samples; ++s)
+        if (pheno[s] == 1)
+            order.push_back(s);
+    const uint32_t kept = (uint32_t)order.size();
+    if (kept == 0)
+        return EPI_EINVAL;
+
     EPI_HIP(hipSetDevice(ctx->device));
     epi_cohort *co = new (std::nothrow) epi_cohort();
     if (!co)
         return EPI_ENOMEM;
     co->ctx = ctx;
-    co->N = n_samples;
+    co->N = kept;
     co->M = n_snps;
     co->Mp = (n_snps + 63) / 64 * 64;
-    // class split in sample order (GPU/v4.cpp:135-171): phenotype 1 = case
-    std::vector<uint32_t> order;
-    order.reserve(n_samples);
-    for (uint32_t s = 0; s < n_samples; ++s)
-        if (pheno[s] != 1)
-            order.push_back(s);
-    co->n[0] = (uint32_t)order.size();
-    for (uint32_t s = 0; s < n_samples; ++s)
-        if (pheno[s] == 1)
-            order.push_back(s);
-    co->n[1] = n_samples - co->n[0];
+    co->n[0] = n0;
+    co->n[1] = kept - n0;
     for (int c = 0; c < 2; ++c)
         co->W[c] = (co->n[c] + 63) / 64;
 
     int st = EPI_OK;
-    DevBuf dgeno, dorder;
+    DevBuf dgeno, dorder, dbed;
     const size_t gbytes = (size_t)n_samples * n_snps;
+    const size_t bbytes = bed ? (size_t)((n_samples + 3) / 4) * n_snps : 0;
     st = dgeno.reserve(gbytes);
     if (st == EPI_OK)
-        st = dorder.reserve((size_t)n_samples * 4);
+        st = dorder.reserve((size_t)kept * 4);
+    if (st == EPI_OK && bed)
+        st = dbed.reserve(bbytes);
     for (int c = 0; c < 2 && st == EPI_OK; ++c) {
         const size_t pb = (size_t)std::max<uint32_t>(co->W[c], 1) * co->Mp * sizeof(uint4);
         st = co->sm[c].reserve(pb);
@@ -282,11 +301,17 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno
     if (st == EPI_OK)
         st = co->desc.reserve(sizeof(CohortDev));
     hipError_t e = hipSuccess;
-    if (st == EPI_OK)
+    if (st == EPI_OK && bed) {
+        e = hipMemcpyAsync(dbed.ptr, bed, bbytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_bed_unpack((const uint8_t *)dbed.ptr, n_samples, n_snps,
+                                  (uint8_t *)dgeno.ptr, ctx->stream);
+    } else if (st == EPI_OK) {
         e = hipMemcpyAsync(dgeno.ptr, geno, gbytes, hipMemcpyHostToDevice, ctx->stream);
+    }
     if (st == EPI_OK && e == hipSuccess)
-        e = hipMemcpyAsync(dorder.ptr, order.data(), (size_t)n_samples * 4,
-                           hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(dorder.ptr, order.data(), (size_t)kept * 4, hipMemcpyHostToDevice,
+                           ctx->stream);
     if (st == EPI_OK && e == hipSuccess)
         e = launch_pack((const uint8_t *)dgeno.ptr, n_samples, n_snps, co->Mp,
                         (const uint32_t *)dorder.ptr, co->n[0], co->n[1],
@@ -321,7 +346,18 @@ extern "C" int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_
         return EPI_EINVAL;
     *out = nullptr;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    return epi::cohort_create_locked(ctx, geno, pheno, n_samples, n_snps, out);
+    return epi::cohort_create_locked(ctx, geno, nullptr, pheno, n_samples, n_snps, out);
+}
+
+extern "C" int epi_cohort_create_bed(epi_ctx *ctx, const uint8_t *bed, const uint8_t *pheno,
+                                     uint32_t n_samples, uint32_t n_snps, epi_cohort **out)
+{
+    if (out)
+        *out = nullptr;
+    if (!ctx || !bed || !pheno || !out || !epi::cohort_shape_ok(n_samples, n_snps))
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return epi::cohort_create_locked(ctx, nullptr, bed, pheno, n_samples, n_snps, out);
 }
 
 CohortDev epi_cohort::host_desc() const

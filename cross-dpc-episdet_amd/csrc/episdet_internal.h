@@ -116,6 +116,8 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
                        const uint32_t *order, uint32_t n0, uint32_t n1,
                        uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1,
                        uint32_t W0, uint32_t W1, hipStream_t s);
+hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t *geno,
+                             hipStream_t s);
 hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &host_cohort,
                         hipStream_t s);
 bool scan_config_supported(const ScanConfig &cfg);

@@ -146,6 +146,47 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
 }
 
 // ------------------------------------------------------------------------
+// PLINK 1 .bed body -> sample-major genotype bytes.  Row s (SNP s) holds
+// bpr bytes; sample t sits in bits 2(t%4)..2(t%4)+1 of byte t/4 with codes
+// 00 homozygous first allele -> 2, 10 heterozygous -> 1, 11 homozygous
+// second allele -> 0, 01 missing -> 3 (neither plane: counted as 2 by the
+// packer, like any value other than 0 or 1).  Threads run over SNPs
+// fastest, so the byte writes of a wavefront are contiguous.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+bed_unpack_kernel(const uint8_t *__restrict__ bed, uint32_t bpr, uint32_t N, uint32_t M,
+                  uint8_t *__restrict__ geno)
+{
+    const size_t t = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (t >= (size_t)bpr * M)
+        return;
+    const uint32_t s = (uint32_t)(t % M);
+    const uint32_t b = (uint32_t)(t / M);
+    const uint32_t v = bed[(size_t)s * bpr + b];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t smp = 4 * b + q;
+        if (smp < N) {
+            const uint32_t code = (v >> (2 * q)) & 3u;
+            // 0 -> 2, 1 -> 3, 2 -> 1, 3 -> 0
+            geno[(size_t)smp * M + s] = (uint8_t)((0x1Eu >> (2 * code)) & 3u);
+        }
+    }
+}
+
+hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t *geno,
+                             hipStream_t s)
+{
+    const uint32_t bpr = (N + 3) / 4;
+    const size_t total = (size_t)bpr * M;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(bed_unpack_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s,
+                       bed, bpr, N, M, geno);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
 // pair and single margins
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict__ cd)
@@ -317,8 +358,10 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
         }
         if (item >= n_items)
             break;
-        if (!DUMP)
-            W.gthr = topk::load_thr(P.thr, cohort);
+        if (!DUMP) {
+            W.gthr = fminf(W.gthr, topk::load_thr(P.thr, cohort));
+            W.thr_slot = P.thr + cohort;
+        }
 
         const uint32_t kb = rows[r].kb;
         const uint32_t jg = rows[r].jg;

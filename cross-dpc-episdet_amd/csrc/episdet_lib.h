@@ -86,8 +86,9 @@ void tune_load_default(epi_ctx *ctx);
 bool tune_lookup(const epi_ctx *ctx, uint32_t n_samples, ScanConfig &cfg);
 bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps);
 uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps);
-int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
-                         uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
+int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
+                         const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
+                         epi_cohort **out);
 void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<double> &t64);
 int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);

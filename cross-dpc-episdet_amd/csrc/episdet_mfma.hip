@@ -202,8 +202,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         }
         if (item >= n_items)
             break;
-        if (!DUMP)
-            Wt.gthr = topk::load_thr(P.thr, cohort);
+        if (!DUMP) {
+            Wt.gthr = fminf(Wt.gthr, topk::load_thr(P.thr, cohort));
+            Wt.thr_slot = P.thr + cohort;
+        }
 
         const uint32_t K0 = rows[r].kb * 64;
         const uint32_t J0 = rows[r].jg * 64;

@@ -8,11 +8,11 @@
 //                than the wave's current 64th best and than the cohort's
 //                global threshold, appended to the wave's LDS buffer;
 //   wave_compact ranks the buffer (keys are unique) and keeps the best 64,
-//                sorted;
+//                sorted; once 64 are kept it lowers the cohort's global
+//                threshold to their 64th best score;
 //   wave_flush   when a wave leaves a cohort: appends its entries that do not
 //                exceed the cohort's global threshold to the cohort's
-//                candidate buffer (one atomic per wave) and lowers the
-//                threshold to the wave's 64th best score.
+//                candidate buffer (one atomic per wave).
 // The global threshold is only ever the 64th best score of some set of 64
 // real candidates, so it bounds the cohort's true 64th best from above and
 // nothing of the true top 64 is dropped.
@@ -44,6 +44,7 @@ struct WaveTop {
     float ts;    // the wave's 64th best (score, key) once it has 64
     uint64_t tk;
     float gthr;  // cohort-wide score threshold (may be stale, never too low)
+    uint32_t *thr_slot;  // the current cohort's threshold word (null: none)
 
     __device__ void reset()
     {
@@ -51,6 +52,7 @@ struct WaveTop {
         ts = INFINITY;
         tk = kEmptyKey;
         gthr = INFINITY;
+        thr_slot = nullptr;
     }
 };
 
@@ -92,12 +94,20 @@ __device__ __noinline__ Compacted compact_buf(Cand *buf, Cand *keep, int n, floa
     return r;
 }
 
+// Compacts the wave's buffer; once it holds 64 candidates their 64th best
+// score is published as the cohort threshold, so that other wavefronts
+// filter early.
 __device__ __forceinline__ void wave_compact(WaveTop &W, uint32_t lane)
 {
     const Compacted r = compact_buf(W.buf, W.keep, W.cnt, W.ts, W.tk, lane);
     W.cnt = __builtin_amdgcn_readfirstlane(r.cnt);
     W.ts = r.ts;
     W.tk = r.tk;
+    if (W.cnt == kTop && W.thr_slot) {
+        if (lane == 0)
+            atomicMin(W.thr_slot, __float_as_uint(W.ts));
+        W.gthr = fminf(W.gthr, W.ts);
+    }
 }
 
 __device__ __forceinline__ void wave_offer(WaveTop &W, bool valid, float s, uint64_t key,
@@ -132,7 +142,8 @@ __device__ __forceinline__ float load_thr(const uint32_t *thr, uint32_t cohort)
 __device__ void wave_flush(WaveTop &W, const ScanParams &P, uint32_t cohort, uint32_t lane)
 {
     if (W.cnt > 0) {
-        wave_compact(W, lane);  // sorted, at most kTop entries
+        W.thr_slot = P.thr + cohort;
+        wave_compact(W, lane);  // sorted, at most kTop entries; publishes the 64th best
         const float g = load_thr(P.thr, cohort);
         const bool pass = (int)lane < W.cnt && W.buf[lane].s <= g;
         const uint64_t mask = __ballot(pass);
@@ -152,8 +163,6 @@ __device__ void wave_flush(WaveTop &W, const ScanParams &P, uint32_t cohort, uin
                 atomicExch(P.overflow, 1u);
             }
         }
-        if (W.cnt == kTop && lane == 0)
-            atomicMin(P.thr + cohort, __float_as_uint(W.buf[kTop - 1].s));
     }
     W.reset();
 }

@@ -89,19 +89,31 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
 // the table in global memory, 2 j-SNPs per wavefront.
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
 {
+    ScanConfig mfma;
+    mfma.variant = 2;  // int8 MFMA: 64-SNP j- and k-blocks, 4 wavefronts
+    mfma.snps_j = 64;
+    mfma.waves = 4;
+    mfma.words = 1;
+    if (tile && tile->variant == 2)
+        return mfma;
+    // a tile that names a VALU shape (or the VALU variant) asks for VALU
+    const bool valu = tile && (tile->variant == 1 || tile->snps_j || tile->waves);
     ScanConfig cfg;
-    if (tile && tile->variant == 2) {
-        cfg.variant = 2;  // int8 MFMA: 64-SNP j- and k-blocks, 4 wavefronts
-        cfg.snps_j = 64;
-        cfg.waves = 4;
-        cfg.words = 1;
-        return cfg;
-    }
     cfg.variant = 1;
     cfg.words = 1;
     cfg.snps_j = 4;
     cfg.waves = 8;
-    if (!tune_lookup(ctx, n_samples, cfg)) {
+    ScanConfig tuned;
+    if (tune_lookup(ctx, n_samples, tuned)) {
+        if (tuned.variant == 2) {
+            if (!valu)
+                return mfma;
+        } else {
+            cfg = tuned;
+        }
+    } else {
+        // untuned: the larger workgroup when the ln(n!) table takes more
+        // than 64 KiB of LDS, fewer j per wavefront when even that fails
         ScanConfig probe = cfg;
         if (scan_lds_bytes(probe, true, n_samples + 2) > 64 * 1024) {
             probe.waves = 16;
@@ -111,8 +123,6 @@ ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_s
                 cfg.snps_j = 2;
         }
     }
-    if (cfg.variant == 2 && !(tile && tile->variant == 1))
-        return cfg;
     cfg.variant = 1;
     if (tile) {
         if (tile->snps_j)

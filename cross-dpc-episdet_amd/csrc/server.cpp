@@ -119,7 +119,8 @@ void epi_server::complete(Request *r, int status)
 // ctx->mu held.
 int epi_server::admit_locked_ctx(Request *r)
 {
-    int st = cohort_create_locked(ctx, r->geno.data(), r->pheno.data(), r->N, r->M, &r->co);
+    int st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
+                                  &r->co);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg);
     if (st == EPI_OK)

@@ -1,11 +1,12 @@
 // tune.cpp -- tile-shape autotuner and the tile table consulted by scans.
 //
-// The scan kernel is instantiated for snps_j (j SNPs per wavefront) in
-// {1, 2, 4, 8} and waves per workgroup in {4, 8, 16}.  Which shape is
-// fastest depends on the sample count (loop length per item vs. scoring
-// and queue overhead, LDS footprint of the ln(n!) table, register
-// pressure).  epi_autotune times them all on a synthetic cohort; the
-// results are kept per context as a table of (max_samples -> tile).
+// The VALU scan kernel is instantiated for snps_j (j SNPs per wavefront) in
+// {1, 2, 4, 8} and waves per workgroup in {4, 8, 16}; the int8 MFMA kernel
+// has one shape.  Which is fastest depends on the sample count (loop length
+// per item vs. scoring and queue overhead, LDS footprint of the ln(n!)
+// table, register pressure, 64 x 64 block waste on the triangle).
+// epi_autotune times them all on a synthetic cohort; the results are kept
+// per context as a table of (max_samples -> tile).
 #include <hip/hip_runtime.h>
 #include <dlfcn.h>
 
@@ -120,7 +121,7 @@ extern "C" int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, e
 {
     if (!ctx || !best || !cohort_shape_ok(n_samples, n_snps))
         return EPI_EINVAL;
-    const uint32_t M = std::min<uint32_t>(n_snps, 768);
+    const uint32_t M = std::min<uint32_t>(n_snps, 1024);
     std::vector<uint8_t> g((size_t)n_samples * M), p(n_samples);
     EPI_TRY(epi_generate_hwe(0x7475E5u, n_samples, M, 0.05, 0.5, n_samples / 2, g.data(),
                              p.data()));
@@ -132,6 +133,7 @@ extern "C" int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, e
     int st = EPI_OK;
     epi_result ref, r;
     bool have_ref = false;
+    std::vector<epi_tile> shapes;
     for (int sj : {1, 2, 4, 8})
         for (int w : {4, 8, 16}) {
             epi_tile t;
@@ -140,28 +142,40 @@ extern "C" int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, e
             t.snps_j = (uint32_t)sj;
             t.waves = (uint32_t)w;
             t.words = 1;
-            double tmin = 1e300;
-            for (int rep = 0; rep < 3 && st == EPI_OK; ++rep) {
-                st = epi_scan(ctx, co, &t, &r);
-                if (st == EPI_OK)
-                    tmin = std::min(tmin, r.kernel_ms);
-            }
-            if (st != EPI_OK)
-                break;
-            // every shape must give the same answer
-            if (!have_ref) {
-                ref = r;
-                have_ref = true;
-            } else if (r.i != ref.i || r.j != ref.j || r.k != ref.k ||
-                       std::memcmp(&r.score, &ref.score, sizeof(float)) != 0) {
-                st = EPI_EINTERNAL;
-                break;
-            }
-            if (tmin < best_t) {
-                best_t = tmin;
-                best_tile = t;
-            }
+            shapes.push_back(t);
         }
+    {
+        epi_tile t;
+        std::memset(&t, 0, sizeof(t));
+        t.variant = 2;
+        t.snps_j = 64;
+        t.waves = 4;
+        t.words = 1;
+        shapes.push_back(t);
+    }
+    for (const epi_tile &t : shapes) {
+        double tmin = 1e300;
+        for (int rep = 0; rep < 3 && st == EPI_OK; ++rep) {
+            st = epi_scan(ctx, co, &t, &r);
+            if (st == EPI_OK)
+                tmin = std::min(tmin, r.kernel_ms);
+        }
+        if (st != EPI_OK)
+            break;
+        // every shape must give the same answer
+        if (!have_ref) {
+            ref = r;
+            have_ref = true;
+        } else if (r.i != ref.i || r.j != ref.j || r.k != ref.k ||
+                   std::memcmp(&r.score, &ref.score, sizeof(float)) != 0) {
+            st = EPI_EINTERNAL;
+            break;
+        }
+        if (tmin < best_t) {
+            best_t = tmin;
+            best_tile = t;
+        }
+    }
     epi_cohort_destroy(co);
     if (st != EPI_OK)
         return st;

@@ -29,6 +29,8 @@ EPI_ECANCELED = -5
 EPI_EAGAIN = -6
 EPI_ENOENT = -7
 EPI_EINTERNAL = -8
+EPI_EIO = -9
+EPI_EFORMAT = -10
 EPI_TOPK = 64
 
 _u32 = ctypes.c_uint32
@@ -105,6 +107,12 @@ _PROTOS = [
     ("epi_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(EpiResult)]),
     ("epi_cancel", ctypes.c_int, [_VP, ctypes.c_uint64]),
     ("epi_server_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(EpiServerStats)]),
+    ("epi_cohort_create_bed", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(_VP)]),
+    ("epi_read_plink", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
+                                      _VP, _VP]),
+    ("epi_write_plink", ctypes.c_int, [ctypes.c_char_p, _VP, _VP, _u32, _u32]),
+    ("epi_read_raw", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, ctypes.POINTER(_u32),
+                                    _VP, _VP]),
     ("epi_generate_reference", ctypes.c_int, [_u32, _u32, _VP, _VP]),
     ("epi_generate_hwe", ctypes.c_int, [ctypes.c_uint64, _u32, _u32, ctypes.c_double,
                                         ctypes.c_double, _u32, _VP, _VP]),
@@ -162,6 +170,42 @@ def _tile(tile) -> Optional[EpiTile]:
     for k, v in dict(tile).items():
         setattr(t, k, int(v))
     return t
+
+
+# ---------------------------------------------------------------- file input
+def read_plink(prefix: str):
+    """Reads <prefix>.bed/.bim/.fam (epi_read_plink): returns (bed, pheno),
+    bed [n_snps][(n_samples+3)//4] uint8, pheno [n_samples] (0 control,
+    1 case, 255 missing)."""
+    n, m = _u32(), _u32()
+    pre = os.fsencode(prefix)
+    _check(lib().epi_read_plink(pre, ctypes.byref(n), ctypes.byref(m), None, None),
+           "epi_read_plink")
+    bed = np.empty((m.value, (n.value + 3) // 4), np.uint8)
+    pheno = np.empty(n.value, np.uint8)
+    _check(lib().epi_read_plink(pre, ctypes.byref(n), ctypes.byref(m), _ptr(bed), _ptr(pheno)),
+           "epi_read_plink")
+    return bed, pheno
+
+
+def write_plink(prefix: str, geno: np.ndarray, pheno: np.ndarray) -> None:
+    """Writes sample-major genotypes as <prefix>.bed/.bim/.fam (epi_write_plink)."""
+    geno = np.ascontiguousarray(geno, dtype=np.uint8)
+    pheno = np.ascontiguousarray(pheno, dtype=np.uint8)
+    _check(lib().epi_write_plink(os.fsencode(prefix), _ptr(geno), _ptr(pheno), geno.shape[0],
+                                 geno.shape[1]), "epi_write_plink")
+
+
+def read_raw(geno_path: str, pheno_path: str, n_samples: int):
+    """Reads raw sample-major genotype and phenotype bytes (epi_read_raw)."""
+    m = _u32()
+    gp, pp = os.fsencode(geno_path), os.fsencode(pheno_path)
+    _check(lib().epi_read_raw(gp, pp, n_samples, ctypes.byref(m), None, None), "epi_read_raw")
+    geno = np.empty((n_samples, m.value), np.uint8)
+    pheno = np.empty(n_samples, np.uint8)
+    _check(lib().epi_read_raw(gp, pp, n_samples, ctypes.byref(m), _ptr(geno), _ptr(pheno)),
+           "epi_read_raw")
+    return geno, pheno
 
 
 # ---------------------------------------------------------------- generators
@@ -269,6 +313,10 @@ class Context:
     def cohort(self, geno: np.ndarray, pheno: np.ndarray) -> "Cohort":
         return Cohort(self, geno, pheno)
 
+    def cohort_bed(self, bed: np.ndarray, pheno: np.ndarray, n_samples: int) -> "Cohort":
+        """Cohort from a PLINK 1 .bed body ([n_snps][(n_samples+3)//4] bytes)."""
+        return Cohort(self, None, pheno, bed=bed, n_samples=n_samples)
+
     def scan(self, cohort: "Cohort", tile=None) -> ScanResult:
         r = EpiResult()
         t = _tile(tile)
@@ -300,10 +348,11 @@ class Context:
         ms = ctypes.c_double()
         _check(lib().epi_autotune(self.handle, n_samples, n_snps, ctypes.byref(t), ctypes.byref(ms)),
                "epi_autotune")
-        return {"snps_j": t.snps_j, "waves": t.waves}, float(ms.value)
+        return {"variant": t.variant, "snps_j": t.snps_j, "waves": t.waves}, float(ms.value)
 
     def tune_set(self, table):
-        """table: [(max_samples, {"snps_j": J, "waves": W}), ...] in increasing order."""
+        """table: [(max_samples, {"variant": V, "snps_j": J, "waves": W}), ...] in
+        increasing order of max_samples (variant 1 = VALU, 2 = int8 MFMA)."""
         n = len(table)
         ms = np.array([int(m) for m, _ in table], np.uint32)
         tiles = (EpiTile * max(n, 1))()
@@ -314,7 +363,7 @@ class Context:
     def tune_get(self, n_samples: int):
         t = EpiTile()
         _check(lib().epi_tune_get(self.handle, n_samples, ctypes.byref(t)), "epi_tune_get")
-        return {"snps_j": t.snps_j, "waves": t.waves}
+        return {"variant": t.variant, "snps_j": t.snps_j, "waves": t.waves}
 
     def server(self, **config) -> "Server":
         return Server(self, **config)
@@ -332,17 +381,31 @@ class Context:
 
 
 class Cohort:
-    """A bit-packed cohort resident in device memory (epi_cohort_create)."""
+    """A bit-packed cohort resident in device memory (epi_cohort_create, or
+    epi_cohort_create_bed with ``bed=``)."""
 
-    def __init__(self, ctx: Context, geno: np.ndarray, pheno: np.ndarray):
-        geno = np.ascontiguousarray(geno, dtype=np.uint8)
+    def __init__(self, ctx: Context, geno: Optional[np.ndarray], pheno: np.ndarray,
+                 bed: Optional[np.ndarray] = None, n_samples: int = 0):
         pheno = np.ascontiguousarray(pheno, dtype=np.uint8)
-        if geno.ndim != 2 or pheno.shape != (geno.shape[0],):
-            raise EpiError(EPI_EINVAL, "geno must be [n_samples][n_snps], pheno [n_samples]")
-        self.n_samples, self.n_snps = geno.shape
         h = ctypes.c_void_p()
-        _check(lib().epi_cohort_create(ctx.handle, _ptr(geno), _ptr(pheno), self.n_samples,
-                                       self.n_snps, ctypes.byref(h)), "epi_cohort_create")
+        if bed is not None:
+            bed = np.ascontiguousarray(bed, dtype=np.uint8)
+            if (bed.ndim != 2 or pheno.shape != (n_samples,) or
+                    bed.shape[1] != (n_samples + 3) // 4):
+                raise EpiError(EPI_EINVAL, "bed must be [n_snps][(n_samples+3)//4], "
+                                           "pheno [n_samples]")
+            self.n_snps = bed.shape[0]
+            self.n_samples = int(np.count_nonzero(pheno <= 1))  # others are left out
+            _check(lib().epi_cohort_create_bed(ctx.handle, _ptr(bed), _ptr(pheno), n_samples,
+                                               self.n_snps, ctypes.byref(h)),
+                   "epi_cohort_create_bed")
+        else:
+            geno = np.ascontiguousarray(geno, dtype=np.uint8)
+            if geno.ndim != 2 or pheno.shape != (geno.shape[0],):
+                raise EpiError(EPI_EINVAL, "geno must be [n_samples][n_snps], pheno [n_samples]")
+            self.n_samples, self.n_snps = geno.shape
+            _check(lib().epi_cohort_create(ctx.handle, _ptr(geno), _ptr(pheno), self.n_samples,
+                                           self.n_snps, ctypes.byref(h)), "epi_cohort_create")
         self.handle = h
 
     @property

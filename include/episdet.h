@@ -45,6 +45,8 @@ extern "C" {
 #define EPI_EAGAIN (-6)     /* result not ready yet (epi_poll) */
 #define EPI_ENOENT (-7)     /* unknown request id */
 #define EPI_EINTERNAL (-8)  /* internal consistency check failed */
+#define EPI_EIO (-9)        /* a file cannot be opened, read or written */
+#define EPI_EFORMAT (-10)   /* malformed input file */
 
 #define EPI_TOPK 64
 
@@ -186,6 +188,39 @@ int epi_poll(epi_server *server, uint64_t request_id, int32_t timeout_ms,
              epi_result *result);
 int epi_cancel(epi_server *server, uint64_t request_id);
 int epi_server_get_stats(epi_server *server, epi_server_stats *stats);
+
+/* On-disk input ----------------------------------------------------------
+ * No counterpart in the reference, which only generates its data
+ * (GPU/v4.cpp:81-110, called at :219). */
+
+/* Cohort from a PLINK 1 .bed body: n_snps rows (SNP-major) of
+ * (n_samples + 3) / 4 bytes; sample t in bits 2(t%4)..2(t%4)+1 of byte t/4,
+ * codes 00 homozygous first allele (genotype 2), 10 heterozygous (1),
+ * 11 homozygous second allele (0), 01 missing (counts as 2, like any value
+ * other than 0/1 in epi_cohort_create).  The body is unpacked on the
+ * device.  pheno: 0 control, 1 case; samples with any other value are left
+ * out (EPI_EINVAL if none is left). */
+int epi_cohort_create_bed(epi_ctx *ctx, const uint8_t *bed, const uint8_t *pheno,
+                          uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
+/* Reads <prefix>.fam (samples; 6th column 2 -> case 1, 1 -> control 0,
+ * anything else -> 0xFF, missing), <prefix>.bim (SNPs) and <prefix>.bed
+ * (SNP-major only, magic 6c 1b 01).  Call first with bed = pheno = NULL to
+ * learn n_samples and n_snps; then again with bed of
+ * n_snps * ((n_samples + 3) / 4) bytes and pheno of n_samples bytes
+ * (n_samples and n_snps must still hold the counts the first call gave). */
+int epi_read_plink(const char *prefix, uint32_t *n_samples, uint32_t *n_snps,
+                   uint8_t *bed, uint8_t *pheno);
+/* Writes a PLINK 1 fileset from sample-major genotypes (0 -> 11, 1 -> 10,
+ * 2 -> 00, other -> 01 missing) and phenotypes (1 -> 2 case, other -> 1). */
+int epi_write_plink(const char *prefix, const uint8_t *geno, const uint8_t *pheno,
+                    uint32_t n_samples, uint32_t n_snps);
+/* Raw input: geno_path holds n_samples x n_snps uint8 genotypes,
+ * sample-major (the reference's A[N][M], GPU/v4.cpp:86-97), pheno_path
+ * n_samples uint8 phenotypes (B[N], :99-108); n_snps = file size /
+ * n_samples.  Call with geno = pheno = NULL to learn n_snps, then with
+ * buffers of n_samples * n_snps and n_samples bytes. */
+int epi_read_raw(const char *geno_path, const char *pheno_path, uint32_t n_samples,
+                 uint32_t *n_snps, uint8_t *geno, uint8_t *pheno);
 
 /* Data generators ------------------------------------------------------- */
 /* The reference generator: srand(100), rand()%3, N/2 cases

@@ -20,6 +20,10 @@
  *                           pinned by distributional checks (HWE genotype
  *                           frequencies, exact class sizes, planted triplet
  *                           recovery).
+ *   orc_bed_decode / orc_bed_encode -- no reference counterpart (published
+ *                           PLINK 1 format); pinned by hand-decoded
+ *                           known-answer bytes, encode/decode round trips
+ *                           and the sample-major scan of the same cohort.
  *
  * Build: see oracle/Makefile (gcc, -ffp-contract=off, OpenMP).
  */
@@ -482,4 +486,32 @@ uint64_t orc_rdtsc(void)
 #else
     return 0;
 #endif
+}
+
+/* ---------------------------------------------------------------------- */
+/* PLINK 1 .bed body (published format; no reference counterpart).       */
+/* ---------------------------------------------------------------------- */
+void orc_bed_decode(const uint8_t *bed, int64_t N, int64_t M, uint8_t *A)
+{
+    static const uint8_t geno_of_code[4] = {2, 3, 1, 0};
+    const int64_t bpr = (N + 3) / 4;
+    for (int64_t j = 0; j < M; ++j)
+        for (int64_t s = 0; s < N; ++s) {
+            const unsigned code = (bed[j * bpr + s / 4] >> (2 * (s % 4))) & 3u;
+            A[s * M + j] = geno_of_code[code];
+        }
+}
+
+void orc_bed_encode(const uint8_t *A, int64_t N, int64_t M, uint8_t *bed)
+{
+    const int64_t bpr = (N + 3) / 4;
+    for (int64_t j = 0; j < M; ++j) {
+        for (int64_t b = 0; b < bpr; ++b)
+            bed[j * bpr + b] = 0;
+        for (int64_t s = 0; s < N; ++s) {
+            const uint8_t g = A[s * M + j];
+            const unsigned code = g == 0 ? 3u : g == 1 ? 2u : g == 2 ? 0u : 1u;
+            bed[j * bpr + s / 4] |= (uint8_t)(code << (2 * (s % 4)));
+        }
+    }
 }

@@ -97,6 +97,16 @@ int64_t orc_scan_all_scores(const orc_packed *p, const float *T, int nthreads,
 /* Combinatorial rank of i<j<k in lexicographic order over C(M,3). */
 int64_t orc_rank3(int64_t M, int64_t i, int64_t j, int64_t k);
 
+/* PLINK 1 .bed body (SNP-major), restated from the published PLINK 1
+ * binary format; the reference itself has no file input (it generates its
+ * data, GPU/v4.cpp:219).  Row j holds (N+3)/4 bytes; sample s sits in bits
+ * 2(s%4)..2(s%4)+1 of byte s/4.  decode: 00 -> 2 (homozygous first
+ * allele), 01 -> 3 (missing), 10 -> 1, 11 -> 0 into sample-major A[N][M].
+ * encode: the inverse, any value other than 0/1/2 -> 01; unused high bits
+ * of a row's last byte are 0. */
+void orc_bed_decode(const uint8_t *bed, int64_t N, int64_t M, uint8_t *A);
+void orc_bed_encode(const uint8_t *A, int64_t N, int64_t M, uint8_t *bed);
+
 /* x86 time-stamp counter (for converting reference cycle counts). */
 uint64_t orc_rdtsc(void);
 

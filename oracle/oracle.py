@@ -63,6 +63,8 @@ def lib():
         L.orc_rank3.argtypes = [_i64, _i64, _i64, _i64]
         L.orc_rank3.restype = _i64
         L.orc_rdtsc.restype = ctypes.c_uint64
+        L.orc_bed_decode.argtypes = [_VP, _i64, _i64, _VP]
+        L.orc_bed_encode.argtypes = [_VP, _i64, _i64, _VP]
         _lib = L
     return _lib
 
@@ -157,6 +159,26 @@ class PackedCohort:
         out = np.empty(M * (M - 1) * (M - 2) // 6, np.float32)
         lib().orc_scan_all_scores(ctypes.byref(self.p), _p(self.T), nthreads, _p(out))
         return out
+
+
+def bed_decode(bed, N):
+    """PLINK 1 .bed body [M][(N+3)//4] -> sample-major genotypes [N][M]
+    (00 -> 2, 01 -> 3 missing, 10 -> 1, 11 -> 0)."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    M = bed.shape[0]
+    assert bed.shape == (M, (N + 3) // 4)
+    A = np.empty((N, M), np.uint8)
+    lib().orc_bed_decode(_p(bed), N, M, _p(A))
+    return A
+
+
+def bed_encode(A):
+    """Sample-major genotypes [N][M] -> PLINK 1 .bed body [M][(N+3)//4]."""
+    A = np.ascontiguousarray(A, dtype=np.uint8)
+    N, M = A.shape
+    bed = np.empty((M, (N + 3) // 4), np.uint8)
+    lib().orc_bed_encode(_p(A), N, M, _p(bed))
+    return bed
 
 
 def rank3(M, i, j, k):

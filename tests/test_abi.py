@@ -72,12 +72,28 @@ def test_generator_argument_checks(epi):
 
 @pytest.mark.parametrize("args", [[], ["10"], ["10", "20"], ["x", "20", "32"], ["10", "2", "32"],
                                   ["10", "20", "0"], ["0", "20", "32"], ["10", "20", "32", "--bogus"],
-                                  ["10", "20", "32", "--cases", "11"]])
+                                  ["10", "20", "32", "--cases", "11"],
+                                  ["10", "20", "32", "--top", "65"],
+                                  ["10", "20", "32", "--variant", "3"],
+                                  ["--bfile"], ["--bfile", "x", "--hwe", "1"],
+                                  ["--bfile", "x", "--cases", "3"],
+                                  ["--raw", "g", "p"], ["--raw", "g", "p", "0"],
+                                  ["--raw", "g", "p", "x"]])
 def test_cli_rejects_malformed_arguments(epi, args):
     if not os.path.exists(epi.CLI_PATH):
         pytest.skip("bin/v4 not built")
     r = subprocess.run([epi.CLI_PATH] + args, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_cli_reports_unreadable_input(epi, tmp_path):
+    # input is read before any device work, so this runs without a GPU
+    if not os.path.exists(epi.CLI_PATH):
+        pytest.skip("bin/v4 not built")
+    for args in (["--bfile", str(tmp_path / "absent")],
+                 ["--raw", str(tmp_path / "g"), str(tmp_path / "p"), "10"]):
+        r = subprocess.run([epi.CLI_PATH] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "reading input failed" in r.stderr, r.stderr
 
 
 def test_context_without_gpu_fails_loudly(epi):
@@ -91,6 +107,6 @@ def test_context_without_gpu_fails_loudly(epi):
 
 
 def test_strerror_names_every_code(epi):
-    for code in range(0, -9, -1):
+    for code in range(0, -11, -1):
         s = epi.strerror(code)
         assert s and "unknown" not in s

@@ -131,16 +131,27 @@ def test_server_poll_unknown_and_timeout(epi, ctx):
         assert r is not None and r.n_snps == 800
 
 
-def test_autotune_and_tile_table(epi, ctx):
-    tile, ms = ctx.autotune(4096, 400)
-    say("autotune 4096 x 400:", tile, ms)
-    assert tile["snps_j"] in (1, 2, 4, 8) and tile["waves"] in (4, 8, 16) and ms > 0
-    ctx.tune_set([(2048, {"snps_j": 2, "waves": 4}), (1 << 20, tile)])
-    assert ctx.tune_get(1000) == {"snps_j": 2, "waves": 4}
-    assert ctx.tune_get(5000) == tile
-    g, p = epi.generate_hwe(2, 1500, 100, 700)
-    with ctx.cohort(g, p) as co:
-        a = ctx.scan(co)
-        b = ctx.scan(co, tile={"snps_j": 8, "waves": 16})
-    assert same(a, b)
-    ctx.tune_set([])
+def test_autotune_and_tile_table(epi):
+    # a context of its own: the tile table is per context and is replaced here
+    with epi.Context(0) as ctx:
+        tile, ms = ctx.autotune(4096, 400)
+        say("autotune 4096 x 400:", tile, ms)
+        assert ms > 0
+        if tile["variant"] == 2:
+            assert tile["snps_j"] == 64 and tile["waves"] == 4
+        else:
+            assert tile["variant"] == 1
+            assert tile["snps_j"] in (1, 2, 4, 8) and tile["waves"] in (4, 8, 16)
+        ctx.tune_set([(2048, {"variant": 1, "snps_j": 2, "waves": 4}), (1 << 20, tile)])
+        assert ctx.tune_get(1000) == {"variant": 1, "snps_j": 2, "waves": 4}
+        assert ctx.tune_get(5000) == tile
+        ctx.tune_set([(1 << 20, {"variant": 2})])
+        assert ctx.tune_get(5000) == {"variant": 2, "snps_j": 64, "waves": 4}
+        g, p = epi.generate_hwe(2, 1500, 100, 700)
+        with ctx.cohort(g, p) as co:
+            a = ctx.scan(co)  # the table's MFMA entry
+            b = ctx.scan(co, tile={"snps_j": 8, "waves": 16})  # an explicit VALU shape
+            c = ctx.scan(co, tile={"variant": 1})
+        assert same(a, b) and same(a, c)
+        ctx.tune_set([])
+        assert ctx.tune_get(1000)["variant"] == 1  # untuned default
