@@ -11,12 +11,17 @@
 // cohort expansion of the bit planes (E[c][snp][plane][sample]).
 //
 // Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
-// 4 wavefronts.  Per stage of 64 samples the workgroup stages the j-block
+// 4 wavefronts.  Per stage of 128 samples the workgroup stages the j-block
 // and k-block bytes (and SNP i's) in LDS; wavefront w computes the 64 rows
 // (16 j x 4 (a,b)) of its quarter against all 128 columns (64 k x 2 c):
-// 2 x 4 MFMA tiles of 32 x 32 per class, 256 int32 accumulators.  The
-// epilogue is the VALU kernel's: exact 27-cell reconstruction from the
-// margins, K2 in the reference order, per-wavefront top-64.
+// 2 x 4 MFMA tiles of 32 x 32 per class, 256 int32 accumulators.
+// Pipelining: two LDS stage buffers, so one barrier per stage; the next
+// stage's bytes are loaded into registers a whole stage ahead and written
+// to the other buffer while this stage's MFMAs run; within a stage the
+// operands of the next 32-sample step are read from LDS while the current
+// step's MFMAs run.  The epilogue is the VALU kernel's: exact 27-cell
+// reconstruction from the margins, K2 in the reference order, per-wavefront
+// top-64.
 #include <math.h>
 
 #include "episdet_cells.h"
@@ -109,12 +114,16 @@ constexpr int kRowPitch = kStage + 16;  // LDS bytes per staged (snp, plane) row
 constexpr int kStageLoads = 2 * 64 * 2 * kStage / 16 / (kMfWaves * 64);  // uint4 per thread
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 
-struct MfLds {
-    Cand cand[kMfWaves][kWaveCap + kTop];
-    uint32_t misc[4];
+struct MfStage {
     uint8_t jrows[kStageRows * kRowPitch];  // j-block bytes of the stage
     uint8_t krows[kStageRows * kRowPitch];  // k-block bytes of the stage
     uint8_t irow[2 * kStage];               // SNP i, both planes
+};
+
+struct MfLds {
+    Cand cand[kMfWaves][kWaveCap + kTop];
+    uint32_t misc[4];
+    MfStage stage[2];  // double buffer: one is read while the other is filled
 };
 
 __host__ __device__ constexpr size_t mf_lds_fixed() { return (sizeof(MfLds) + 15) & ~(size_t)15; }
@@ -139,6 +148,22 @@ __device__ __forceinline__ void mf_load_stage(v4i (&pre)[kStageLoads], v4i &pre_
     }
     if (tid < 16)
         pre_i = *(const v4i *)(E + ((size_t)i * 2 + (tid >> 3)) * Lc + s0 + 16 * (tid & 7u));
+}
+
+// Writes a stage loaded by mf_load_stage into one LDS stage buffer.
+__device__ __forceinline__ void mf_store_stage(MfStage &S, const v4i (&pre)[kStageLoads],
+                                               const v4i &pre_i, uint32_t tid)
+{
+#pragma unroll
+    for (int q = 0; q < kStageLoads; ++q) {
+        const uint32_t idx = tid + 256u * q;
+        const uint32_t blk = idx >> 10;
+        const uint32_t rowi = (idx >> 3) & 127u;
+        const uint32_t chunk = idx & 7u;
+        *(v4i *)((blk ? S.krows : S.jrows) + rowi * kRowPitch + 16 * chunk) = pre[q];
+    }
+    if (tid < 16)
+        *(v4i *)(S.irow + (tid >> 3) * kStage + 16 * (tid & 7u)) = pre_i;
 }
 
 template <bool TLDS, bool DUMP>
@@ -171,12 +196,30 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     const uint32_t half = lane >> 5;   // which 16 bytes of a 32-sample step
     const uint32_t l32 = lane & 31u;   // A row / B column within a tile
 
+    // lane-constant LDS offsets of the MFMA operands (bytes, within a stage
+    // buffer, for the first 32-sample step)
+    const uint32_t a_lane = (l32 >> 1) & 1u;  // g_i plane of this lane's A rows
+    uint32_t ja_off[2];
+#pragma unroll
+    for (int fa = 0; fa < 2; ++fa) {
+        // A row l32 of tile fa = (j_local, a, b), j_local = 16w + 8fa + l32/4
+        const uint32_t jl = 16 * wave + 8 * fa + (l32 >> 2);
+        ja_off[fa] = (jl * 2 + (l32 & 1u)) * kRowPitch + 16 * half;
+    }
+    // B column l32 of tile fb = (k_local, c), k_local = 16fb + l32/2: row 32fb + l32
+    const uint32_t kb_off = l32 * kRowPitch + 16 * half;
+    const uint32_t ei_off = a_lane * kStage + 16 * half;
+
+    // Work-queue items: the next item is claimed while this one runs.
+    if (tid == 0)
+        L.misc[0] = atomicAdd(&P.counters[0], 1u);
+    __syncthreads();
+    uint32_t item = mf::rfl(L.misc[0]);
     for (;;) {
-        if (tid == 0)
-            L.misc[0] = atomicAdd(&P.counters[0], 1u);
-        __syncthreads();
-        const uint32_t item = mf::rfl(L.misc[0]);
-        __syncthreads();
+        __syncthreads();  // every wavefront has read misc[0]
+        uint32_t claimed = 0;
+        if (tid == 0 && item < n_items)
+            claimed = atomicAdd(&P.counters[0], 1u);
         uint32_t cohort = 0xFFFFFFFFu, u = 0;
         const EPI_CONST Row *rows = nullptr;
         if (item < n_items) {
@@ -224,59 +267,72 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     for (int e = 0; e < 16; ++e)
                         acc[c][fa][fb][e] = 0;
 
-        // Stage loop: the next stage's bytes are loaded into registers while
-        // the current stage (in LDS) feeds the MFMAs.
+        // Stage loop (see the pipelining note at the top of this file).
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const uint8_t *E = C->E[c];
             const uint32_t Lc = C->EL[c];  // bytes per (snp, plane) row, a multiple of kStage
             if (Lc == 0)
                 continue;
+            const uint32_t n_st = Lc / kStage;
             // thread t stages uint4 number idx = t + 256 q of the stage:
             // block (j/k) = idx / 1024, row (snp_local, plane) = (idx / 8) % 128,
             // 16-byte chunk = idx % 8
             v4i pre[kStageLoads];
             v4i pre_i = {0, 0, 0, 0};
+            // Both buffers are free here: the previous stage loop ended with
+            // a barrier and the epilogue does not touch them.
             mf_load_stage(pre, pre_i, E, Lc, 0u, J0, K0, i, tid);
-            for (uint32_t s0 = 0; s0 < Lc; s0 += kStage) {
-                __syncthreads();  // the previous stage is no longer read
+            mf_store_stage(L.stage[0], pre, pre_i, tid);
+            if (n_st > 1)
+                mf_load_stage(pre, pre_i, E, Lc, kStage, J0, K0, i, tid);
+            __syncthreads();
+            for (uint32_t st = 0; st < n_st; ++st) {
+                const MfStage &S = L.stage[st & 1];
+                v4i ei[kStage / 32];
 #pragma unroll
-                for (int q = 0; q < kStageLoads; ++q) {
-                    const uint32_t idx = tid + 256u * q;
-                    const uint32_t blk = idx >> 10;
-                    const uint32_t rowi = (idx >> 3) & 127u;
-                    const uint32_t chunk = idx & 7u;
-                    *(v4i *)((blk ? L.krows : L.jrows) + rowi * kRowPitch + 16 * chunk) = pre[q];
-                }
-                if (tid < 16)
-                    *(v4i *)(L.irow + (tid >> 3) * kStage + 16 * (tid & 7u)) = pre_i;
-                if (s0 + kStage < Lc)
-                    mf_load_stage(pre, pre_i, E, Lc, s0 + kStage, J0, K0, i, tid);
-                __syncthreads();
+                for (int kk = 0; kk < kStage / 32; ++kk)
+                    ei[kk] = lds16(S.irow + ei_off + 32 * kk);
+                v4i aj[2][2], bk[2][4];
+#pragma unroll
+                for (int fa = 0; fa < 2; ++fa)
+                    aj[0][fa] = lds16(S.jrows + ja_off[fa]);
+#pragma unroll
+                for (int fb = 0; fb < 4; ++fb)
+                    bk[0][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb);
 #pragma unroll
                 for (int kk = 0; kk < kStage / 32; ++kk) {
-                    const uint32_t off = kk * 32 + 16 * half;
-                    // A: row l32 of tile fa = (j_local, a, b) with j_local = 16w + 8fa + l32/4
-                    v4i A[2];
-#pragma unroll
-                    for (int fa = 0; fa < 2; ++fa) {
-                        const uint32_t jl = 16 * wave + 8 * fa + (l32 >> 2);
-                        const uint32_t a = (l32 >> 1) & 1u, b = l32 & 1u;
-                        const v4i ej = lds16(L.jrows + (jl * 2 + b) * kRowPitch + off);
-                        const v4i ei = lds16(L.irow + a * kStage + off);
-                        A[fa] = ej & ei;
-                    }
-#pragma unroll
-                    for (int fb = 0; fb < 4; ++fb) {
-                        // B: column l32 of tile fb = (k_local, c) with k_local = 16fb + l32/2
-                        const uint32_t kl = 16 * fb + (l32 >> 1), cc = l32 & 1u;
-                        const v4i Bv = lds16(L.krows + (kl * 2 + cc) * kRowPitch + off);
+                    const int cb = kk & 1, nb = cb ^ 1;
+                    if (kk + 1 < kStage / 32) {  // operands of the next step
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] =
-                                __builtin_amdgcn_mfma_i32_32x32x32_i8(A[fa], Bv, acc[c][fa][fb], 0, 0, 0);
+                            aj[nb][fa] = lds16(S.jrows + ja_off[fa] + 32 * (kk + 1));
+#pragma unroll
+                        for (int fb = 0; fb < 4; ++fb)
+                            bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb + 32 * (kk + 1));
                     }
+                    if (kk == 1 && st + 1 < n_st) {
+                        // fill the other buffer (read by the previous stage,
+                        // released by the barrier that ended it) and start
+                        // loading the stage after it
+                        mf_store_stage(L.stage[(st + 1) & 1], pre, pre_i, tid);
+                        if (st + 2 < n_st)
+                            mf_load_stage(pre, pre_i, E, Lc, (st + 2) * kStage, J0, K0, i, tid);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    v4i A[2];
+#pragma unroll
+                    for (int fa = 0; fa < 2; ++fa)
+                        A[fa] = aj[cb][fa] & ei[kk];
+#pragma unroll
+                    for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+                        for (int fa = 0; fa < 2; ++fa)
+                            acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                A[fa], bk[cb][fb], acc[c][fa][fb], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
+                __syncthreads();  // this buffer is read; the other is written
             }
         }
 
@@ -384,6 +440,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
             }
         }
+        if (tid == 0)
+            L.misc[0] = claimed;
+        __syncthreads();
+        item = mf::rfl(L.misc[0]);
     }
 }
 

@@ -82,11 +82,13 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
     }
 }
 
-// Tile shape: explicit fields of `tile` win; then the autotuned table of
-// the context; otherwise a default that follows the table placement: with
-// the ln(n!) table in LDS, 4 j-SNPs per wavefront fit in 128 VGPRs; a large
-// table in LDS allows one workgroup per CU, so it gets 16 wavefronts; with
-// the table in global memory, 2 j-SNPs per wavefront.
+// Kernel and tile shape: variant 2 in `tile` selects the MFMA kernel; a
+// tile naming the VALU variant or a VALU shape selects the VALU kernel;
+// otherwise the context's tile table (per sample-count bucket) decides.
+// A VALU shape not given by the tile or the table follows the ln(n!) table
+// placement: in LDS, 4 j-SNPs per wavefront fit in 128 VGPRs; a large table
+// in LDS allows one workgroup per CU, so it gets 16 wavefronts; with the
+// table in global memory, 2 j-SNPs per wavefront.
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
 {
     ScanConfig mfma;
@@ -104,16 +106,15 @@ ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_s
     cfg.snps_j = 4;
     cfg.waves = 8;
     ScanConfig tuned;
-    if (tune_lookup(ctx, n_samples, tuned)) {
-        if (tuned.variant == 2) {
-            if (!valu)
-                return mfma;
-        } else {
-            cfg = tuned;
-        }
+    const bool have = tune_lookup(ctx, n_samples, tuned);
+    if (have && tuned.variant == 2 && !valu)
+        return mfma;
+    if (have && tuned.variant != 2) {
+        cfg = tuned;
     } else {
-        // untuned: the larger workgroup when the ln(n!) table takes more
-        // than 64 KiB of LDS, fewer j per wavefront when even that fails
+        // no tuned VALU shape: the larger workgroup when the ln(n!) table
+        // takes more than 64 KiB of LDS, fewer j per wavefront when even
+        // that fails
         ScanConfig probe = cfg;
         if (scan_lds_bytes(probe, true, n_samples + 2) > 64 * 1024) {
             probe.waves = 16;
