@@ -120,11 +120,28 @@ struct MfStage {
     uint8_t irow[2 * kStage];               // SNP i, both planes
 };
 
+// The margins an item's epilogue reads, staged in LDS over the (then idle)
+// stage buffers.  The j-block is split in two halves of 32 SNPs, one per
+// accumulator row tile fa: half fa holds j_local = 16w + 8fa + r (w < 4,
+// r < 8) as row 8w + r.
+constexpr int kJkPitch = 64 + 1;  // uint4 per JK row (+1: bank offset)
+struct MfMargins {
+    uint4 jk[2][32][kJkPitch];  // pair margins of (j, k), one j half, both classes
+    uint4 ij[2][64];            // pair margins of (i, j)
+    uint4 ik[2][64];            // pair margins of (i, k)
+    uint2 sj[2][64];            // single-SNP counts of the j-block
+    uint2 sk[2][64];            // and of the k-block
+};
+
 struct MfLds {
     Cand cand[kMfWaves][kWaveCap + kTop];
     uint32_t misc[4];
-    MfStage stage[2];  // double buffer: one is read while the other is filled
+    union {
+        MfStage stage[2];  // double buffer: one is read while the other is filled
+        MfMargins mg;      // the epilogue's margins (after the stage loop)
+    };
 };
+static_assert(sizeof(MfMargins) <= 2 * sizeof(MfStage), "margins must fit the stage buffers");
 
 __host__ __device__ constexpr size_t mf_lds_fixed() { return (sizeof(MfLds) + 15) & ~(size_t)15; }
 
@@ -148,6 +165,50 @@ __device__ __forceinline__ void mf_load_stage(v4i (&pre)[kStageLoads], v4i &pre_
     }
     if (tid < 16)
         pre_i = *(const v4i *)(E + ((size_t)i * 2 + (tid >> 3)) * Lc + s0 + 16 * (tid & 7u));
+}
+
+// Stages one j half of the item's (j, k) pair margins, and with half 0 also
+// the (i, j) and (i, k) margins and the single-SNP counts of both blocks.
+// Entries with j >= k or i >= j are read but never used (the pair table
+// holds p < q only; all indices stay below Mp).
+__device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST CohortDev *C,
+                                                 uint32_t i, uint32_t J0, uint32_t K0,
+                                                 uint32_t Mp, int fa, uint32_t tid)
+{
+    constexpr int kLoads = 2 * 32 * 64 / (kMfWaves * 64);  // uint4 per thread
+    uint4 v[kLoads];
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) {
+        const uint32_t idx = tid + 256u * q;  // (c, row, k_local)
+        const uint32_t c = idx >> 11, row = (idx >> 6) & 31u, kl = idx & 63u;
+        const uint32_t jl = 16 * (row >> 3) + 8 * fa + (row & 7u);
+        v[q] = C->pair[c][(size_t)(J0 + jl) * Mp + K0 + kl];
+    }
+    uint4 x = make_uint4(0, 0, 0, 0);
+    uint2 y = make_uint2(0, 0);
+    if (fa == 0) {
+        const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
+        if (tid & 64u)
+            x = C->pair[c][(size_t)i * Mp + K0 + e];
+        else
+            x = C->pair[c][(size_t)i * Mp + J0 + e];
+        y = C->single[c][((tid & 64u) ? K0 : J0) + e];
+    }
+#pragma unroll
+    for (int q = 0; q < kLoads; ++q) {
+        const uint32_t idx = tid + 256u * q;
+        G.jk[idx >> 11][(idx >> 6) & 31u][idx & 63u] = v[q];
+    }
+    if (fa == 0) {
+        const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
+        if (tid & 64u) {
+            G.ik[c][e] = x;
+            G.sk[c][e] = y;
+        } else {
+            G.ij[c][e] = x;
+            G.sj[c][e] = y;
+        }
+    }
 }
 
 // Writes a stage loaded by mf_load_stage into one LDS stage buffer.
@@ -281,7 +342,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             v4i pre[kStageLoads];
             v4i pre_i = {0, 0, 0, 0};
             // Both buffers are free here: the previous stage loop ended with
-            // a barrier and the epilogue does not touch them.
+            // a barrier, and the previous item's epilogue (which reuses them
+            // for margins) ended with the barrier before the next claim.
             mf_load_stage(pre, pre_i, E, Lc, 0u, J0, K0, i, tid);
             mf_store_stage(L.stage[0], pre, pre_i, tid);
             if (n_st > 1)
@@ -346,6 +408,12 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         const bool odd = (lane & 1u) != 0;
 #pragma unroll 1
         for (int t = 0; t < 8; ++t) {
+            if ((t & 3) == 0) {
+                // the stage buffers (or the previous half) are no longer read
+                __syncthreads();
+                mf_stage_margins(L.mg, C, i, J0, K0, Mp, t >> 2, tid);
+                __syncthreads();
+            }
             int v[2][16];
             switch (t) {
 #define EPI_MF_COPY(T, FA, FB)                      \
@@ -367,72 +435,85 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             }
             const uint32_t fa = (uint32_t)t >> 2, fb = (uint32_t)t & 3u;
             const uint32_t k = K0 + 16 * fb + (l32 >> 1);
-            int w[2][16];  // the partner lane's values
+            const uint32_t kl = k - K0;
+            int w[2][16];  // the partner lane's values: swap lanes 2m and 2m+1 (DPP)
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
-                    w[c][e] = __shfl_xor(v[c][e], 1);
+                    w[c][e] = __builtin_amdgcn_mov_dpp(v[c][e], 0xB1, 0xF, 0xF, false);
 #pragma unroll
             for (int sl = 0; sl < 2; ++sl) {
                 const uint32_t slot = (odd ? 2u : 0u) + (uint32_t)sl;  // e / 4
                 const uint32_t j = J0 + 16 * wave + 8 * fa + 2 * slot + half;
                 const bool valid = (i < j) && (j < k) && (k < M);
-                float s = INFINITY;
-                if (valid) {
-                    uint32_t x0[27], x1[27];
+                const uint32_t jl = j - J0;
+                // every lane is scored (invalid lanes get clamped table
+                // indices and score +inf): no divergent branch around the
+                // batched table reads
+                uint32_t x[2][27];
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        uint32_t t8[8];
+                for (int c = 0; c < 2; ++c) {
+                    uint32_t t8[8];
 #pragma unroll
-                        for (int ab = 0; ab < 4; ++ab) {
-                            // element of this lane's j-slot: even lanes slots 0,1, odd 2,3
-                            const uint32_t mine = (uint32_t)(odd ? v[c][8 + 4 * sl + ab] : v[c][4 * sl + ab]);
-                            const uint32_t theirs = (uint32_t)(odd ? w[c][8 + 4 * sl + ab] : w[c][4 * sl + ab]);
-                            t8[2 * ab + 0] = odd ? theirs : mine;  // cc = 0 lives in the even lane
-                            t8[2 * ab + 1] = odd ? mine : theirs;
-                        }
-                        const uint4 ij = C->pair[c][(size_t)i * Mp + j];
-                        const uint4 ik = C->pair[c][(size_t)i * Mp + k];
-                        const uint4 jk = C->pair[c][(size_t)j * Mp + k];
-                        const uint2 si = mf::ldc2(C->single[c] + i);
-                        const uint2 sj = C->single[c][j];
-                        const uint2 sk = C->single[c][k];
-                        const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
-                        const uint32_t aik[4] = {ik.x, ik.y, ik.z, ik.w};
-                        const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
-                        const uint32_t asi[2] = {si.x, si.y};
-                        const uint32_t asj[2] = {sj.x, sj.y};
-                        const uint32_t ask[2] = {sk.x, sk.y};
-                        derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n1 : n0, c ? x1 : x0);
+                    for (int ab = 0; ab < 4; ++ab) {
+                        // element of this lane's j-slot: even lanes slots 0,1, odd 2,3
+                        const uint32_t mine = (uint32_t)(odd ? v[c][8 + 4 * sl + ab] : v[c][4 * sl + ab]);
+                        const uint32_t theirs = (uint32_t)(odd ? w[c][8 + 4 * sl + ab] : w[c][4 * sl + ab]);
+                        t8[2 * ab + 0] = odd ? theirs : mine;  // cc = 0 lives in the even lane
+                        t8[2 * ab + 1] = odd ? mine : theirs;
                     }
-                    float sc = 0.0f;
+                    const uint4 ij = L.mg.ij[c][jl];
+                    const uint4 ik = L.mg.ik[c][kl];
+                    const uint4 jk = L.mg.jk[c][8 * wave + 2 * slot + half][kl];
+                    const uint2 si = mf::ldc2(C->single[c] + i);
+                    const uint2 sj = L.mg.sj[c][jl];
+                    const uint2 sk = L.mg.sk[c][kl];
+                    const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
+                    const uint32_t aik[4] = {ik.x, ik.y, ik.z, ik.w};
+                    const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
+                    const uint32_t asi[2] = {si.x, si.y};
+                    const uint32_t asj[2] = {sj.x, sj.y};
+                    const uint32_t ask[2] = {sk.x, sk.y};
+                    derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n1 : n0, x[c]);
+                }
+                // K2 in the reference order (GPU/v4.cpp:579-583,
+                // CPU/v2.cpp:516-519); the table reads of 9 cells are issued
+                // ahead of their ordered sums
+                float sc = 0.0f;
 #pragma unroll
-                    for (int m = 0; m < 27; ++m) {
-                        const uint32_t ia = min(x0[m] + x1[m] + 1, tmax);
-                        const uint32_t ib = min(x0[m], tmax);
-                        const uint32_t id = min(x1[m], tmax);
-                        const float av = TLDS ? Tl[ia] : P.table[ia];
-                        const float bv = TLDS ? Tl[ib] : P.table[ib];
-                        const float dv = TLDS ? Tl[id] : P.table[id];
-                        sc += (av - bv) - dv;
-                        if (!TLDS && (m % 3) == 2)
-                            __builtin_amdgcn_sched_barrier(0);
+                for (int m0 = 0; m0 < 27; m0 += 9) {
+                    float ta[9], tb[9], td[9];
+#pragma unroll
+                    for (int m = 0; m < 9; ++m) {
+                        // exact counts give indices <= N + 1 < table_len; the
+                        // clamp keeps invalid lanes' values inside the table
+                        const uint32_t c0 = x[0][m0 + m], c1 = x[1][m0 + m];
+                        const uint32_t ia = min(c0 + c1 + 1, tmax);
+                        const uint32_t ib = min(c0, tmax);
+                        const uint32_t id = min(c1, tmax);
+                        ta[m] = TLDS ? Tl[ia] : P.table[ia];
+                        tb[m] = TLDS ? Tl[ib] : P.table[ib];
+                        td[m] = TLDS ? Tl[id] : P.table[id];
                     }
-                    s = fabsf(sc);
-                    if (DUMP) {
-                        const uint64_t MM = M;
-                        auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; };
-                        auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
-                        const uint64_t rk = c3(MM) - c3(MM - i) + c2(MM - i - 1) - c2(MM - j) +
-                                            (uint64_t)(k - j - 1);
-                        P.dump_scores[rk] = s;
-                        if (P.dump_counts) {
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                            for (int m = 0; m < 27; ++m) {
-                                P.dump_counts[rk * 54 + m] = x0[m];
-                                P.dump_counts[rk * 54 + 27 + m] = x1[m];
-                            }
+                    for (int m = 0; m < 9; ++m)
+                        sc += (ta[m] - tb[m]) - td[m];
+                }
+                const float s = valid ? fabsf(sc) : INFINITY;
+                if (DUMP && valid) {
+                    const uint64_t MM = M;
+                    auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; };
+                    auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
+                    const uint64_t rk = c3(MM) - c3(MM - i) + c2(MM - i - 1) - c2(MM - j) +
+                                        (uint64_t)(k - j - 1);
+                    P.dump_scores[rk] = s;
+                    if (P.dump_counts) {
+#pragma unroll
+                        for (int m = 0; m < 27; ++m) {
+                            P.dump_counts[rk * 54 + m] = x[0][m];
+                            P.dump_counts[rk * 54 + 27 + m] = x[1][m];
                         }
                     }
                 }
