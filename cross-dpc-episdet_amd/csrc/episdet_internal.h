@@ -34,7 +34,7 @@ constexpr uint64_t kEmptyKey = ~0ull;
 struct CohortDev {
     const uint4 *sm[2];   // SNP-major  [Mp][W[c]]   (uniform loads)
     const uint4 *wm[2];   // word-major [W[c]][Mp]   (per-lane loads)
-    uint4 *pair[2];       // [Mp][Mp] for p < q: {n00, n01, n10, n11}
+    uint4 *pair[2];       // [Mp][Mp], every (p, q): {n00, n01, n10, n11}
     uint2 *single[2];     // [Mp]: {count g0, count g1}
     const uint8_t *E[2];  // MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
     uint32_t EL[2];       // bytes per E row: 64 W[c] rounded up to 128, zero padded

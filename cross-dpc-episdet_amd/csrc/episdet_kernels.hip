@@ -193,10 +193,7 @@ __global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict_
 {
     const uint32_t c = blockIdx.z;
     const uint32_t p = blockIdx.y;
-    const uint32_t qb = blockIdx.x * 256;
-    if (qb + 255 < p)
-        return;
-    const uint32_t q = qb + threadIdx.x;
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     const uint32_t Mp = as_const(cd)->Mp;
     if (q >= Mp)
         return;
@@ -212,9 +209,11 @@ __global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict_
         a10 += __popc(x.z & z.x) + __popc(x.w & z.y);
         a11 += __popc(x.z & z.z) + __popc(x.w & z.w);
     }
-    if (q > p)
-        as_const(cd)->pair[c][(size_t)p * Mp + q] = make_uint4(a00, a01, a10, a11);
-    else if (q == p)
+    // every (p, q), not only p < q: the MFMA kernel's epilogue scores whole
+    // 64 x 64 tiles, and with the full table the counts of its lanes outside
+    // i < j < k are exact too (so every ln(n!) index stays in the table)
+    as_const(cd)->pair[c][(size_t)p * Mp + q] = make_uint4(a00, a01, a10, a11);
+    if (q == p)
         as_const(cd)->single[c][p] = make_uint2(a00, a11);
 }
 

@@ -169,8 +169,8 @@ __device__ __forceinline__ void mf_load_stage(v4i (&pre)[kStageLoads], v4i &pre_
 
 // Stages one j half of the item's (j, k) pair margins, and with half 0 also
 // the (i, j) and (i, k) margins and the single-SNP counts of both blocks.
-// Entries with j >= k or i >= j are read but never used (the pair table
-// holds p < q only; all indices stay below Mp).
+// Entries with j >= k or i >= j belong to lanes that are scored but never
+// offered; all indices stay below Mp.
 __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST CohortDev *C,
                                                  uint32_t i, uint32_t J0, uint32_t K0,
                                                  uint32_t Mp, int fa, uint32_t tid)
@@ -349,6 +349,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             if (n_st > 1)
                 mf_load_stage(pre, pre_i, E, Lc, kStage, J0, K0, i, tid);
             __syncthreads();
+            // The last step's MFMAs of a stage are issued after the barrier
+            // that ends it, together with the next stage's first reads, so
+            // the matrix cores stay busy across the barrier.
+            v4i dA[2], dB[4];  // deferred step: A (masked by SNP i) and B
             for (uint32_t st = 0; st < n_st; ++st) {
                 const MfStage &S = L.stage[st & 1];
                 v4i ei[kStage / 32];
@@ -362,17 +366,26 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
                 for (int fb = 0; fb < 4; ++fb)
                     bk[0][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb);
+                if (st > 0) {  // the previous stage's last step
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int kk = 0; kk < kStage / 32; ++kk) {
-                    const int cb = kk & 1, nb = cb ^ 1;
-                    if (kk + 1 < kStage / 32) {  // operands of the next step
+                    for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            aj[nb][fa] = lds16(S.jrows + ja_off[fa] + 32 * (kk + 1));
+                            acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                                dA[fa], dB[fb], acc[c][fa][fb], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
 #pragma unroll
-                        for (int fb = 0; fb < 4; ++fb)
-                            bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb + 32 * (kk + 1));
-                    }
+                for (int kk = 0; kk < kStage / 32 - 1; ++kk) {
+                    const int cb = kk & 1, nb = cb ^ 1;
+                    // operands of the next step (the last one into dA, dB)
+#pragma unroll
+                    for (int fa = 0; fa < 2; ++fa)
+                        aj[nb][fa] = lds16(S.jrows + ja_off[fa] + 32 * (kk + 1));
+#pragma unroll
+                    for (int fb = 0; fb < 4; ++fb)
+                        bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb + 32 * (kk + 1));
                     if (kk == 1 && st + 1 < n_st) {
                         // fill the other buffer (read by the previous stage,
                         // released by the barrier that ended it) and start
@@ -394,8 +407,21 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                                 A[fa], bk[cb][fb], acc[c][fa][fb], 0, 0, 0);
                     __builtin_amdgcn_sched_barrier(0);
                 }
+                constexpr int kl = kStage / 32 - 1, lb = kl & 1;
+#pragma unroll
+                for (int fa = 0; fa < 2; ++fa)
+                    dA[fa] = aj[lb][fa] & ei[kl];
+#pragma unroll
+                for (int fb = 0; fb < 4; ++fb)
+                    dB[fb] = bk[lb][fb];
                 __syncthreads();  // this buffer is read; the other is written
             }
+#pragma unroll
+            for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+                for (int fa = 0; fa < 2; ++fa)
+                    acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                        dA[fa], dB[fb], acc[c][fa][fb], 0, 0, 0);
         }
 
         // ---- epilogue: lane holds column (k, cc) and rows (j, ab) with
@@ -404,7 +430,6 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         // holds the other cc; even lanes score j-slots 0,1, odd lanes 2,3.
         // One tile at a time (not unrolled) keeps the scoring code single.
         const uint32_t n0 = C->n[0], n1 = C->n[1];
-        const uint32_t tmax = P.table_len - 1;
         const bool odd = (lane & 1u) != 0;
 #pragma unroll 1
         for (int t = 0; t < 8; ++t) {
@@ -448,9 +473,11 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 const uint32_t j = J0 + 16 * wave + 8 * fa + 2 * slot + half;
                 const bool valid = (i < j) && (j < k) && (k < M);
                 const uint32_t jl = j - J0;
-                // every lane is scored (invalid lanes get clamped table
-                // indices and score +inf): no divergent branch around the
-                // batched table reads
+                // every lane is scored (lanes outside i < j < k score +inf):
+                // no divergent branch around the batched table reads.  The
+                // pair table holds every (p, q), so even those lanes' counts
+                // are exact and every index below is at most N + 1, inside
+                // the table.
                 uint32_t x[2][27];
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
@@ -486,12 +513,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     float ta[9], tb[9], td[9];
 #pragma unroll
                     for (int m = 0; m < 9; ++m) {
-                        // exact counts give indices <= N + 1 < table_len; the
-                        // clamp keeps invalid lanes' values inside the table
-                        const uint32_t c0 = x[0][m0 + m], c1 = x[1][m0 + m];
-                        const uint32_t ia = min(c0 + c1 + 1, tmax);
-                        const uint32_t ib = min(c0, tmax);
-                        const uint32_t id = min(c1, tmax);
+                        const uint32_t ib = x[0][m0 + m], id = x[1][m0 + m];
+                        const uint32_t ia = ib + id + 1;
                         ta[m] = TLDS ? Tl[ia] : P.table[ia];
                         tb[m] = TLDS ? Tl[ib] : P.table[ib];
                         td[m] = TLDS ? Tl[id] : P.table[id];
