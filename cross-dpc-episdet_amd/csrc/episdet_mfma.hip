@@ -16,10 +16,11 @@
 // (16 j x 4 (a,b)) of its quarter against all 128 columns (64 k x 2 c):
 // 2 x 4 MFMA tiles of 32 x 32 per class, 256 int32 accumulators.
 // Pipelining: two LDS stage buffers, so one barrier per stage; the next
-// stage's bytes are loaded into registers a whole stage ahead and written
-// to the other buffer while this stage's MFMAs run; within a stage the
-// operands of the next 32-sample step are read from LDS while the current
-// step's MFMAs run.  The epilogue is the VALU kernel's: exact 27-cell
+// stage is copied from global memory straight into the other buffer
+// (global_load_lds, no registers) while this stage's MFMAs run; within a
+// stage the operands of the next 32-sample step are read from LDS while the
+// current step's MFMAs run, and a stage's last step runs after the barrier
+// together with the next stage's first reads.  The epilogue is the VALU kernel's: exact 27-cell
 // reconstruction from the margins, K2 in the reference order, per-wavefront
 // top-64.
 #include <math.h>
@@ -110,15 +111,19 @@ hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, 
 // ------------------------------------------------------------------------
 constexpr int kMfWaves = 4;
 constexpr int kStage = 128;           // samples per LDS stage (four 32-sample MFMA steps)
-constexpr int kRowPitch = kStage + 16;  // LDS bytes per staged (snp, plane) row, padded
-constexpr int kStageLoads = 2 * 64 * 2 * kStage / 16 / (kMfWaves * 64);  // uint4 per thread
+constexpr int kRowBytes = kStage;     // LDS bytes per staged (snp, plane) row
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
+constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
 
+// Staged rows are 128 bytes with no padding; 16-byte slot s of row r holds
+// chunk s ^ (r & 7) of the row, so that the operand reads of 8 consecutive
+// lanes (4 or 8 different rows at one chunk) hit 8 different bank groups.
 struct MfStage {
-    uint8_t jrows[kStageRows * kRowPitch];  // j-block bytes of the stage
-    uint8_t krows[kStageRows * kRowPitch];  // k-block bytes of the stage
-    uint8_t irow[2 * kStage];               // SNP i, both planes
+    uint8_t jrows[kStageRows * kRowBytes];  // j-block bytes of the stage
+    uint8_t krows[kStageRows * kRowBytes];  // k-block bytes of the stage (right after)
+    uint8_t irow[2 * kStage];               // SNP i, both planes, unswizzled
 };
+static_assert(__builtin_offsetof(MfStage, krows) == kStageRows * kRowBytes, "k rows follow j rows");
 
 // The margins an item's epilogue reads, staged in LDS over the (then idle)
 // stage buffers.  The j-block is split in two halves of 32 SNPs, one per
@@ -141,30 +146,62 @@ struct MfLds {
         MfMargins mg;      // the epilogue's margins (after the stage loop)
     };
 };
-static_assert(sizeof(MfMargins) <= 2 * sizeof(MfStage), "margins must fit the stage buffers");
 
 __host__ __device__ constexpr size_t mf_lds_fixed() { return (sizeof(MfLds) + 15) & ~(size_t)15; }
 
 // 16 bytes of LDS as an MFMA operand
 __device__ __forceinline__ v4i lds16(const uint8_t *p) { return *(const v4i *)p; }
 
-// Loads one stage into registers: thread t takes uint4 number idx = t + 256 q,
-// block (j/k) = idx / 1024, row (snp_local, plane) = (idx / 8) % 128,
-// 16-byte chunk = idx % 8; threads 0..15 also take SNP i's two rows.
-__device__ __forceinline__ void mf_load_stage(v4i (&pre)[kStageLoads], v4i &pre_i,
-                                              const uint8_t *E, uint32_t Lc, uint32_t s0,
-                                              uint32_t J0, uint32_t K0, uint32_t i, uint32_t tid)
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+
+// One 16-byte-per-lane copy from global memory into LDS (lane L writes
+// bytes 16L..16L+15 after the wave-uniform LDS address `lds`).  Written as
+// asm so that the compiler's wait-count model, which counts these copies
+// as LDS operations too and then waits for every outstanding LDS read,
+// keeps exact counts for the operand reads; completion is awaited
+// explicitly (s_waitcnt vmcnt) before the barrier that publishes a stage.
+// The instruction has no register result.
+__device__ __forceinline__ void lds_copy16(const uint8_t *src, uint32_t lds)
 {
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(lds)
+                 : "memory", "m0");
+}
+
+// LDS byte address of a pointer into LDS.
+__device__ __forceinline__ uint32_t lds_addr(const void *p)
+{
+    return (uint32_t)(uintptr_t)(const lds_u8 *)p;
+}
+
+// Copies one stage from global memory straight into an LDS stage buffer
+// (global_load_lds_dwordx4: lane L of a copy writes bytes 16L..16L+15 of a
+// 1 KiB block given by M0).  The j and k rows (32 KiB) are 32 copies of 8
+// rows each, 8 per wavefront; lane L takes row L/8 of its copy and reads
+// the global chunk (L%8) ^ (L/8) that the swizzle puts in its slot.  SNP
+// i's 256 bytes are one 4-lane copy per wavefront.  Every wavefront issues
+// kStageCopies + 1 copies, so one vmcnt value covers a stage on all.
+// `stage` is the LDS byte address of the MfStage to fill (wave-uniform).
+__device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *E, uint32_t Lc,
+                                               uint32_t s0, uint32_t J0, uint32_t K0, uint32_t i,
+                                               uint32_t wave, uint32_t lane)
+{
+    const uint32_t rsub = lane >> 3, chunk = (lane & 7u) ^ rsub;
 #pragma unroll
-    for (int q = 0; q < kStageLoads; ++q) {
-        const uint32_t idx = tid + 256u * q;
-        const uint32_t blk = idx >> 10;
-        const uint32_t rowi = (idx >> 3) & 127u;
+    for (int q = 0; q < kStageCopies; ++q) {
+        // rows first..first+7 of the stage's 256 (j rows, then k rows)
+        const uint32_t first = (wave * kStageCopies + q) * 8;
+        const uint32_t blk = first >> 7, rowi = (first & 127u) + rsub;
         const uint32_t snp = (blk ? K0 : J0) + (rowi >> 1);
-        pre[q] = *(const v4i *)(E + ((size_t)snp * 2 + (rowi & 1)) * Lc + s0 + 16 * (idx & 7u));
+        const uint8_t *src = E + ((size_t)snp * 2 + (rowi & 1u)) * Lc + s0 + 16 * chunk;
+        lds_copy16(src, stage + first * kRowBytes);  // krows follows jrows
     }
-    if (tid < 16)
-        pre_i = *(const v4i *)(E + ((size_t)i * 2 + (tid >> 3)) * Lc + s0 + 16 * (tid & 7u));
+    if (lane < 4) {
+        const uint32_t e = 4 * wave + lane;  // 16-byte chunk of SNP i's two rows
+        const uint8_t *src = E + ((size_t)i * 2 + (e >> 3)) * Lc + s0 + 16 * (e & 7u);
+        lds_copy16(src, stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
+    }
 }
 
 // Stages one j half of the item's (j, k) pair margins, and with half 0 also
@@ -211,27 +248,12 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
     }
 }
 
-// Writes a stage loaded by mf_load_stage into one LDS stage buffer.
-__device__ __forceinline__ void mf_store_stage(MfStage &S, const v4i (&pre)[kStageLoads],
-                                               const v4i &pre_i, uint32_t tid)
-{
-#pragma unroll
-    for (int q = 0; q < kStageLoads; ++q) {
-        const uint32_t idx = tid + 256u * q;
-        const uint32_t blk = idx >> 10;
-        const uint32_t rowi = (idx >> 3) & 127u;
-        const uint32_t chunk = idx & 7u;
-        *(v4i *)((blk ? S.krows : S.jrows) + rowi * kRowPitch + 16 * chunk) = pre[q];
-    }
-    if (tid < 16)
-        *(v4i *)(S.irow + (tid >> 3) * kStage + 16 * (tid & 7u)) = pre_i;
-}
-
 template <bool TLDS, bool DUMP>
 __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanParams P)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     MfLds &L = *reinterpret_cast<MfLds *>(lds_raw);
+    const uint32_t stage_lds[2] = {lds_addr(&L.stage[0]), lds_addr(&L.stage[1])};
     float *Tl = reinterpret_cast<float *>(lds_raw + mf_lds_fixed());
 
     const uint32_t tid = threadIdx.x;
@@ -258,17 +280,25 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     const uint32_t l32 = lane & 31u;   // A row / B column within a tile
 
     // lane-constant LDS offsets of the MFMA operands (bytes, within a stage
-    // buffer, for the first 32-sample step)
+    // buffer): row offsets, and per 32-sample step the swizzled slot
     const uint32_t a_lane = (l32 >> 1) & 1u;  // g_i plane of this lane's A rows
     uint32_t ja_off[2];
 #pragma unroll
     for (int fa = 0; fa < 2; ++fa) {
         // A row l32 of tile fa = (j_local, a, b), j_local = 16w + 8fa + l32/4
         const uint32_t jl = 16 * wave + 8 * fa + (l32 >> 2);
-        ja_off[fa] = (jl * 2 + (l32 & 1u)) * kRowPitch + 16 * half;
+        ja_off[fa] = (jl * 2 + (l32 & 1u)) * kRowBytes;
     }
     // B column l32 of tile fb = (k_local, c), k_local = 16fb + l32/2: row 32fb + l32
-    const uint32_t kb_off = l32 * kRowPitch + 16 * half;
+    const uint32_t kb_off = l32 * kRowBytes;
+    const uint32_t jx = (2 * (l32 >> 2) + (l32 & 1u)) & 7u;  // row & 7 of this lane's A rows
+    const uint32_t kx = l32 & 7u;                            // row & 7 of its B rows
+    uint32_t ja_slot[kStage / 32], kb_slot[kStage / 32];
+#pragma unroll
+    for (int kk = 0; kk < kStage / 32; ++kk) {
+        ja_slot[kk] = 16 * ((2 * kk + half) ^ jx);
+        kb_slot[kk] = 16 * ((2 * kk + half) ^ kx);
+    }
     const uint32_t ei_off = a_lane * kStage + 16 * half;
 
     // Work-queue items: the next item is claimed while this one runs.
@@ -336,18 +366,11 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             if (Lc == 0)
                 continue;
             const uint32_t n_st = Lc / kStage;
-            // thread t stages uint4 number idx = t + 256 q of the stage:
-            // block (j/k) = idx / 1024, row (snp_local, plane) = (idx / 8) % 128,
-            // 16-byte chunk = idx % 8
-            v4i pre[kStageLoads];
-            v4i pre_i = {0, 0, 0, 0};
             // Both buffers are free here: the previous stage loop ended with
             // a barrier, and the previous item's epilogue (which reuses them
             // for margins) ended with the barrier before the next claim.
-            mf_load_stage(pre, pre_i, E, Lc, 0u, J0, K0, i, tid);
-            mf_store_stage(L.stage[0], pre, pre_i, tid);
-            if (n_st > 1)
-                mf_load_stage(pre, pre_i, E, Lc, kStage, J0, K0, i, tid);
+            mf_issue_stage(stage_lds[0], E, Lc, 0u, J0, K0, i, wave, lane);
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             // The last step's MFMAs of a stage are issued after the barrier
             // that ends it, together with the next stage's first reads, so
@@ -359,13 +382,16 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
                 for (int kk = 0; kk < kStage / 32; ++kk)
                     ei[kk] = lds16(S.irow + ei_off + 32 * kk);
+                if (st + 1 < n_st)  // the next stage, into the buffer the last one read
+                    mf_issue_stage(stage_lds[(st + 1) & 1], E, Lc, (st + 1) * kStage, J0, K0, i,
+                                   wave, lane);
                 v4i aj[2][2], bk[2][4];
 #pragma unroll
                 for (int fa = 0; fa < 2; ++fa)
-                    aj[0][fa] = lds16(S.jrows + ja_off[fa]);
+                    aj[0][fa] = lds16(S.jrows + ja_off[fa] + ja_slot[0]);
 #pragma unroll
                 for (int fb = 0; fb < 4; ++fb)
-                    bk[0][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb);
+                    bk[0][fb] = lds16(S.krows + kb_off + 32 * kRowBytes * fb + kb_slot[0]);
                 if (st > 0) {  // the previous stage's last step
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -382,18 +408,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     // operands of the next step (the last one into dA, dB)
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
-                        aj[nb][fa] = lds16(S.jrows + ja_off[fa] + 32 * (kk + 1));
+                        aj[nb][fa] = lds16(S.jrows + ja_off[fa] + ja_slot[kk + 1]);
 #pragma unroll
                     for (int fb = 0; fb < 4; ++fb)
-                        bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowPitch * fb + 32 * (kk + 1));
-                    if (kk == 1 && st + 1 < n_st) {
-                        // fill the other buffer (read by the previous stage,
-                        // released by the barrier that ended it) and start
-                        // loading the stage after it
-                        mf_store_stage(L.stage[(st + 1) & 1], pre, pre_i, tid);
-                        if (st + 2 < n_st)
-                            mf_load_stage(pre, pre_i, E, Lc, (st + 2) * kStage, J0, K0, i, tid);
-                    }
+                        bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowBytes * fb + kb_slot[kk + 1]);
                     __builtin_amdgcn_sched_barrier(0);
                     v4i A[2];
 #pragma unroll
@@ -414,7 +432,9 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
                 for (int fb = 0; fb < 4; ++fb)
                     dB[fb] = bk[lb][fb];
-                __syncthreads();  // this buffer is read; the other is written
+                // this buffer is read and the next stage has landed in the other
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
             }
 #pragma unroll
             for (int fb = 0; fb < 4; ++fb)
