@@ -116,8 +116,12 @@ constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
 
 // Staged rows are 128 bytes with no padding; 16-byte slot s of row r holds
-// chunk s ^ (r & 7) of the row, so that the operand reads of 8 consecutive
-// lanes (4 or 8 different rows at one chunk) hit 8 different bank groups.
+// chunk s ^ ((r >> 1) & 7) of the row.  LDS serves 16 lanes of a 16-byte
+// read per cycle from 64 banks (256 bytes): even rows fall on one half of
+// the banks and odd rows on the other, and the swizzle spreads the 8 even
+// (or odd) rows of 16 consecutive rows over the 8 slots of their half, so
+// the operand reads (16 lanes: up to 16 different rows at one chunk) are
+// free of bank conflicts.
 struct MfStage {
     uint8_t jrows[kStageRows * kRowBytes];  // j-block bytes of the stage
     uint8_t krows[kStageRows * kRowBytes];  // k-block bytes of the stage (right after)
@@ -179,7 +183,7 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p)
 // (global_load_lds_dwordx4: lane L of a copy writes bytes 16L..16L+15 of a
 // 1 KiB block given by M0).  The j and k rows (32 KiB) are 32 copies of 8
 // rows each, 8 per wavefront; lane L takes row L/8 of its copy and reads
-// the global chunk (L%8) ^ (L/8) that the swizzle puts in its slot.  SNP
+// the global chunk that the swizzle puts in its slot L%8.  SNP
 // i's 256 bytes are one 4-lane copy per wavefront.  Every wavefront issues
 // kStageCopies + 1 copies, so one vmcnt value covers a stage on all.
 // `stage` is the LDS byte address of the MfStage to fill (wave-uniform).
@@ -187,12 +191,13 @@ __device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *E,
                                                uint32_t s0, uint32_t J0, uint32_t K0, uint32_t i,
                                                uint32_t wave, uint32_t lane)
 {
-    const uint32_t rsub = lane >> 3, chunk = (lane & 7u) ^ rsub;
+    const uint32_t rsub = lane >> 3;
 #pragma unroll
     for (int q = 0; q < kStageCopies; ++q) {
         // rows first..first+7 of the stage's 256 (j rows, then k rows)
         const uint32_t first = (wave * kStageCopies + q) * 8;
         const uint32_t blk = first >> 7, rowi = (first & 127u) + rsub;
+        const uint32_t chunk = (lane & 7u) ^ ((rowi >> 1) & 7u);
         const uint32_t snp = (blk ? K0 : J0) + (rowi >> 1);
         const uint8_t *src = E + ((size_t)snp * 2 + (rowi & 1u)) * Lc + s0 + 16 * chunk;
         lds_copy16(src, stage + first * kRowBytes);  // krows follows jrows
@@ -291,8 +296,11 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     }
     // B column l32 of tile fb = (k_local, c), k_local = 16fb + l32/2: row 32fb + l32
     const uint32_t kb_off = l32 * kRowBytes;
-    const uint32_t jx = (2 * (l32 >> 2) + (l32 & 1u)) & 7u;  // row & 7 of this lane's A rows
-    const uint32_t kx = l32 & 7u;                            // row & 7 of its B rows
+    // swizzle keys (row >> 1) & 7 of this lane's A and B rows
+    // (A row 2 j_local + b: (row >> 1) & 7 = j_local & 7 = (l32 / 4) & 7;
+    //  B row 32 fb + l32: (l32 / 2) & 7)
+    const uint32_t jx = (l32 >> 2) & 7u;
+    const uint32_t kx = (l32 >> 1) & 7u;
     uint32_t ja_slot[kStage / 32], kb_slot[kStage / 32];
 #pragma unroll
     for (int kk = 0; kk < kStage / 32; ++kk) {
