@@ -38,8 +38,9 @@ ORACLE = os.path.join(ROOT, "oracle")
 
 # MI355X integer VALU issue rate: 256 CU x 64 lanes x 2.4 GHz (lane-ops/s)
 VALU_PEAK_OPS = 256 * 64 * 2.4e9
-# MI355X dense int8 matrix-core rate (ops/s)
+# MI355X dense int8 and FP4 matrix-core rates (ops/s)
 MFMA_I8_PEAK_OPS = 5.0e15
+MFMA_FP4_PEAK_OPS = 10.1e15
 # Reference work per triplet-sample, the roofline's algorithmic unit: 27 cells
 # x (3-input AND + popcount) per 32 samples on the VALU, 27 int8 MACs (54
 # ops) as a matrix product.  Both kernels here popcount (or multiply) only
@@ -51,10 +52,15 @@ WORK = {
         "unit_text": "54 int32 lane-ops per (triplet x 32 samples), the reference's 27-cell count"},
     2: {"bound": "mfma", "peak": MFMA_I8_PEAK_OPS, "ops": 54.0, "executed_ops": 16.0,
         "unit_text": "54 int8 ops (27 MACs) per triplet-sample, the reference's 27-cell count"},
+    3: {"bound": "mfma", "peak": MFMA_FP4_PEAK_OPS, "ops": 54.0, "executed_ops": 16.0,
+        "unit_text": "54 FP4 ops (27 MACs of 0/1 e2m1 values, exact FP32 sums) per "
+                     "triplet-sample, the reference's 27-cell count"},
 }
 
 
-KERNELS = {1: "scan_kernel (VALU AND + popcount)", 2: "mfma_scan_kernel (int8 MFMA)"}
+KERNELS = {1: "scan_kernel (VALU AND + popcount)", 2: "mfma_scan_kernel (int8 MFMA)",
+           3: "mfma_scan_kernel (FP4 MFMA)"}
+DTYPES = {1: "u32", 2: "i8", 3: "fp4"}
 
 
 def load_binding():
@@ -224,7 +230,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-snps", type=int, default=1024)
-    ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 VALU, 2 int8 MFMA")
+    ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 VALU, 2 int8 MFMA, 3 FP4 MFMA")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the one-scan comparison of the VALU and MFMA kernels")
     ap.add_argument("--no-batch", action="store_true")
@@ -266,16 +272,18 @@ def main():
             if not args.no_compare:
                 # one scan with each formulation of the 8 popcounted cells
                 compare = {}
-                for name, var in (("valu", 1), ("mfma", 2)):
+                for name, var in (("valu", 1), ("mfma", 2), ("fp4", 3)):
                     rv = ctx.scan(co, {"variant": var})
                     kern = rv.kernel_ms / 1e3
                     compare[name] = {"kernel_ms": rv.kernel_ms, "scan_ms": rv.scan_ms,
                                      "triplets_x_samples_per_s": trip(M) * N / kern,
                                      "identical": same_result(rv, r)}
                     log(f"{name}: kernel {rv.kernel_ms:.1f} ms")
-                # MFMA work executed: 8 int8 MACs per triplet-sample (2 ops each)
-                compare["mfma"]["int8_tops"] = trip(M) * N * 16 / (compare["mfma"]["kernel_ms"] / 1e3) / 1e12
-                compare["mfma"]["mfma_util"] = compare["mfma"]["int8_tops"] * 1e12 / MFMA_I8_PEAK_OPS
+                # MFMA work executed: 8 MACs per triplet-sample (2 ops each)
+                for name, peak in (("mfma", MFMA_I8_PEAK_OPS), ("fp4", MFMA_FP4_PEAK_OPS)):
+                    tops = trip(M) * N * 16 / (compare[name]["kernel_ms"] / 1e3) / 1e12
+                    compare[name]["executed_tops"] = tops
+                    compare[name]["mfma_util"] = tops * 1e12 / peak
         device = ctx.device_name
         capacity = M * (M - 1) * (M - 2) // 6 * N * args.steps / elapsed
         batched = serving = None
@@ -320,7 +328,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32" if variant == 1 else "i8",
+        "dtype": DTYPES[variant],
         "data": "synthetic",
         "config": {"workload": f"exhaustive 3-way K2 scan, {M} SNPs x {N} samples "
                                f"({cases} cases), HWE MAF U[0.05,0.5], seed {args.seed}",

@@ -219,6 +219,7 @@ epi_cohort::~epi_cohort()
         pair[c].release();
         single[c].release();
         E[c].release();
+        F[c].release();
     }
     desc.release();
 }
@@ -371,6 +372,8 @@ CohortDev epi_cohort::host_desc() const
         h.single[c] = (uint2 *)single[c].ptr;
         h.E[c] = (const uint8_t *)E[c].ptr;
         h.EL[c] = EL[c];
+        h.F[c] = (const uint8_t *)F[c].ptr;
+        h.FL[c] = FL[c];
         h.W[c] = W[c];
         h.n[c] = n[c];
     }

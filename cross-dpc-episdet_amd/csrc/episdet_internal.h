@@ -36,8 +36,10 @@ struct CohortDev {
     const uint4 *wm[2];   // word-major [W[c]][Mp]   (per-lane loads)
     uint4 *pair[2];       // [Mp][Mp], every (p, q): {n00, n01, n10, n11}
     uint2 *single[2];     // [Mp]: {count g0, count g1}
-    const uint8_t *E[2];  // MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
+    const uint8_t *E[2];  // int8 MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
     uint32_t EL[2];       // bytes per E row: 64 W[c] rounded up to 128, zero padded
+    const uint8_t *F[2];  // FP4 MFMA variant: e2m1 nibbles 0 / 1.0 [Mp][2 planes][FL[c]], or null
+    uint32_t FL[2];       // bytes per F row: 32 W[c] rounded up to 128, zero padded
     uint32_t W[2];        // 64-bit words per class
     uint32_t n[2];        // class sizes
     uint32_t M, Mp;       // SNPs, padded to a multiple of 64
@@ -104,7 +106,7 @@ struct Rescored {
 
 // Scan kernel configuration (template instance selector).
 struct ScanConfig {
-    int variant = 1;  // 1 VALU AND+popcount, 2 int8 MFMA
+    int variant = 1;  // 1 VALU AND+popcount, 2 int8 MFMA, 3 FP4 MFMA
     int snps_j = 4;   // VALU: 1, 2, 4, 8 j SNPs per wavefront; MFMA: 64 (j-block)
     int waves = 8;    // wavefronts per workgroup
     int words = 1;    // 64-bit sample words per loop step
@@ -121,14 +123,17 @@ hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t
 hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &host_cohort,
                         hipStream_t s);
 bool scan_config_supported(const ScanConfig &cfg);
-// MFMA variant (episdet_mfma.hip)
+// MFMA variants (episdet_mfma.hip); fp4 selects the FP4 kernel (variant 3)
 hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, uint8_t *E,
                          hipStream_t s);
+hipError_t launch_expand_fp4(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t FL, uint8_t *F,
+                             hipStream_t s);
 size_t mfma_lds_bytes(bool table_lds, uint32_t table_len);
-hipError_t mfma_occupancy(bool table_lds, bool dump, size_t lds_bytes, int *blocks_per_cu);
-hipError_t mfma_allow_lds(bool table_lds, bool dump, size_t lds_bytes);
-hipError_t launch_mfma_scan(bool table_lds, bool dump, const ScanParams &p, uint32_t grid,
-                            hipStream_t s);
+hipError_t mfma_occupancy(bool fp4, bool table_lds, bool dump, size_t lds_bytes,
+                          int *blocks_per_cu);
+hipError_t mfma_allow_lds(bool fp4, bool table_lds, bool dump, size_t lds_bytes);
+hipError_t launch_mfma_scan(bool fp4, bool table_lds, bool dump, const ScanParams &p,
+                            uint32_t grid, hipStream_t s);
 hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump,
                           size_t lds_bytes, int *blocks_per_cu);
 size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len);

@@ -453,7 +453,7 @@ size_t lds_total(bool table_lds, uint32_t table_len)
 
 size_t scan_lds_bytes(const ScanConfig &cfg, bool table_lds, uint32_t table_len)
 {
-    if (cfg.variant == 2)
+    if (cfg.variant >= 2)
         return mfma_lds_bytes(table_lds, table_len);
     switch (cfg.waves) {
     case 4: return lds_total<4>(table_lds, table_len);
@@ -499,7 +499,7 @@ static ScanFn pick(const ScanConfig &cfg, bool tl, bool dump)
 
 bool scan_config_supported(const ScanConfig &cfg)
 {
-    if (cfg.variant == 2)
+    if (cfg.variant == 2 || cfg.variant == 3)
         return cfg.snps_j == 64 && cfg.waves == 4 && cfg.words == 1;
     return pick(cfg, false, false) != nullptr;
 }
@@ -507,8 +507,8 @@ bool scan_config_supported(const ScanConfig &cfg)
 hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes,
                           int *blocks_per_cu)
 {
-    if (cfg.variant == 2)
-        return mfma_occupancy(table_lds, dump, lds_bytes, blocks_per_cu);
+    if (cfg.variant >= 2)
+        return mfma_occupancy(cfg.variant == 3, table_lds, dump, lds_bytes, blocks_per_cu);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;
@@ -518,8 +518,8 @@ hipError_t scan_occupancy(const ScanConfig &cfg, bool table_lds, bool dump, size
 
 hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size_t lds_bytes)
 {
-    if (cfg.variant == 2)
-        return mfma_allow_lds(table_lds, dump, lds_bytes);
+    if (cfg.variant >= 2)
+        return mfma_allow_lds(cfg.variant == 3, table_lds, dump, lds_bytes);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;
@@ -530,8 +530,8 @@ hipError_t scan_allow_lds(const ScanConfig &cfg, bool table_lds, bool dump, size
 hipError_t launch_scan(const ScanConfig &cfg, bool table_lds, bool dump, const ScanParams &p,
                        uint32_t grid, hipStream_t s)
 {
-    if (cfg.variant == 2)
-        return launch_mfma_scan(table_lds, dump, p, grid, s);
+    if (cfg.variant >= 2)
+        return launch_mfma_scan(cfg.variant == 3, table_lds, dump, p, grid, s);
     ScanFn f = pick(cfg, table_lds, dump);
     if (!f)
         return hipErrorInvalidValue;

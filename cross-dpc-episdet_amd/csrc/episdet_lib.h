@@ -117,8 +117,9 @@ int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool du
                 LaunchPlan &plan);
 int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg);
 int enqueue_pairs(epi_ctx *ctx, epi_cohort *co);
-// Builds the cohort's 0/1-byte planes (MFMA variant) if they do not exist.
-int ensure_expanded(epi_ctx *ctx, epi_cohort *co);
+// Builds the cohort's expanded planes for an MFMA variant (2: 0/1 bytes,
+// 3: e2m1 nibbles) if they do not exist.
+int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant);
 // Uploads the round tables (rows [ranges[b].first, ranges[b].second) of
 // jobs[b]) and enqueues queue reset, scan kernel and merge.
 int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
@@ -159,8 +160,10 @@ struct epi_cohort {
     uint32_t n[2] = {0, 0};
     uint32_t W[2] = {0, 0};
     epi::DevBuf sm[2], wm[2], pair[2], single[2], desc;
-    epi::DevBuf E[2];  // 0/1-byte planes for the MFMA variant, built on first use
+    epi::DevBuf E[2];  // 0/1-byte planes for the int8 MFMA variant, built on first use
     uint32_t EL[2] = {0, 0};  // their row pitch in bytes
+    epi::DevBuf F[2];  // e2m1 nibble planes for the FP4 MFMA variant, built on first use
+    uint32_t FL[2] = {0, 0};  // their row pitch in bytes
     uint64_t bytes = 0;
     epi::CohortDev host_desc() const;
     ~epi_cohort();

@@ -1,14 +1,25 @@
-// episdet_mfma.hip -- int8 matrix-core formulation of the 8 popcounted
-// cells (the alternative to the VALU AND+popcount scan kernel).
+// episdet_mfma.hip -- matrix-core formulations of the 8 popcounted cells
+// (the alternative to the VALU AND+popcount scan kernel).
 //
 // For one class and a fixed SNP i, the counts the VALU kernel popcounts,
 //   t[a][b][c](i,j,k) = sum_s [g_i(s)=a] [g_j(s)=b] [g_k(s)=c],  a,b,c in {0,1},
 // are a matrix product over samples s:
-//   A[(j,a,b)][s] = [g_i(s)=a] & [g_j(s)=b]      (0/1 bytes)
-//   B[s][(k,c)]   = [g_k(s)=c]                    (0/1 bytes)
-//   C = A B       with exact int32 accumulation,
-// computed with v_mfma_i32_32x32x32_i8.  The 0/1 bytes come from a per-
-// cohort expansion of the bit planes (E[c][snp][plane][sample]).
+//   A[(j,a,b)][s] = [g_i(s)=a] & [g_j(s)=b]      (0/1)
+//   B[s][(k,c)]   = [g_k(s)=c]                    (0/1)
+//   C = A B
+// in one of two number formats, both exact:
+//   int8 (variant 2): 0/1 bytes, v_mfma_i32_32x32x32_i8, int32 accumulation;
+//   FP4  (variant 3): e2m1 nibbles 0 and 1.0 (0x2), two samples per byte,
+//        v_mfma_scale_f32_32x32x64_f8f6f4 (both block scales 2^0), FP32
+//        accumulation.  Every product is 0 or 1 and every partial sum an
+//        integer below 2^24 (class sizes are below 2^21), so the FP32 sums are
+//        exact; the instruction does twice the int8 instruction's MACs in the
+//        same issue time, and the operands take half the bytes.
+// The operand bytes come from a per-cohort expansion of the bit planes
+// (E[c] or F[c], [snp][plane][sample]).  Both formats use one LDS layout:
+// 128-byte rows per stage, 32 bytes per MFMA step (32 samples as bytes, 64
+// as nibbles); A and B map a byte position to the same sample, so the sum
+// over the K dimension covers each sample once whatever that mapping is.
 //
 // Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
 // 4 wavefronts.  Per stage of 128 samples the workgroup stages the j-block
@@ -106,11 +117,83 @@ hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, 
     return hipGetLastError();
 }
 
+// bit planes -> e2m1 nibbles: F[c] row (snp, plane) holds FL bytes (32 W_c
+// rounded up to a whole 128-byte stage); sample t is nibble t % 8 of dword
+// t / 8, 0x2 (1.0) when bit t of the plane is set, else 0; padding is 0.
+// One thread per (snp, 64-bit word slot, plane) writes 32 bytes.
+__global__ void __launch_bounds__(256)
+expand_fp4_kernel(const uint4 *__restrict__ sm, uint32_t Mp, uint32_t W, uint32_t FL,
+                  uint8_t *__restrict__ F)
+{
+    const size_t t = blockIdx.x * (size_t)256 + threadIdx.x;
+    const uint32_t WS = FL / 32;  // word slots per row
+    const size_t total = (size_t)Mp * WS * 2;
+    if (t >= total)
+        return;
+    const uint32_t plane = (uint32_t)(t & 1);
+    const size_t sw = t >> 1;  // snp * WS + w
+    const uint32_t w = (uint32_t)(sw % WS);
+    const size_t snp = sw / WS;
+    const uint4 v = w < W ? sm[snp * W + w] : make_uint4(0, 0, 0, 0);
+    const uint32_t lo = plane ? v.z : v.x;
+    const uint32_t hi = plane ? v.w : v.y;
+    uint32_t out[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        // spread 8 bits to bit 0 of 8 nibbles, then make each a 1.0 (0x2)
+        uint32_t x = ((q < 4 ? lo : hi) >> (8 * (q & 3))) & 0xFFu;
+        x = (x | (x << 12)) & 0x000F000Fu;
+        x = (x | (x << 6)) & 0x03030303u;
+        x = (x | (x << 3)) & 0x11111111u;
+        out[q] = x << 1;
+    }
+    uint4 *dst = (uint4 *)(F + (snp * 2 + plane) * (size_t)FL + (size_t)w * 32);
+    dst[0] = make_uint4(out[0], out[1], out[2], out[3]);
+    dst[1] = make_uint4(out[4], out[5], out[6], out[7]);
+}
+
+hipError_t launch_expand_fp4(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t FL, uint8_t *F,
+                             hipStream_t s)
+{
+    const size_t total = (size_t)Mp * (FL / 32) * 2;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(expand_fp4_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s,
+                       sm, Mp, W, FL, F);
+    return hipGetLastError();
+}
+
+// One MFMA step of a 32 x 32 tile: 16 operand bytes per lane, as 16 int8
+// samples (int32 accumulators) or 32 e2m1 samples (FP32 accumulators; the
+// instruction reads 4 of the 8 operand registers for e2m1, format code 4,
+// and scale exponent 127 = 2^0).
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ v16i mf_mma(v4i a, v4i b, v16i c)
+{
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v16f mf_mma(v4i a, v4i b, v16f c)
+{
+    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+    const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+template <bool FP4>
+struct MfAcc {
+    typedef v16i type;
+};
+template <>
+struct MfAcc<true> {
+    typedef v16f type;
+};
+
 // ------------------------------------------------------------------------
 // the MFMA scan kernel
 // ------------------------------------------------------------------------
 constexpr int kMfWaves = 4;
-constexpr int kStage = 128;           // samples per LDS stage (four 32-sample MFMA steps)
+constexpr int kStage = 128;           // bytes per row per LDS stage (four 32-byte MFMA steps)
 constexpr int kRowBytes = kStage;     // LDS bytes per staged (snp, plane) row
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
@@ -253,7 +336,7 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
     }
 }
 
-template <bool TLDS, bool DUMP>
+template <bool TLDS, bool DUMP, bool FP4>
 __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanParams P)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -355,7 +438,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         const EPI_CONST CohortDev *C = mf::as_const(P.cohorts) + cohort;
         const uint32_t M = C->M, Mp = C->Mp;
 
-        v16i acc[2][2][4];
+        typename MfAcc<FP4>::type acc[2][2][4];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -369,8 +452,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         // Stage loop (see the pipelining note at the top of this file).
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const uint8_t *E = C->E[c];
-            const uint32_t Lc = C->EL[c];  // bytes per (snp, plane) row, a multiple of kStage
+            const uint8_t *E = FP4 ? C->F[c] : C->E[c];
+            const uint32_t Lc = FP4 ? C->FL[c] : C->EL[c];  // bytes per (snp, plane) row, a multiple of kStage
             if (Lc == 0)
                 continue;
             const uint32_t n_st = Lc / kStage;
@@ -406,8 +489,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                                dA[fa], dB[fb], acc[c][fa][fb], 0, 0, 0);
+                            acc[c][fa][fb] = mf_mma(dA[fa], dB[fb], acc[c][fa][fb]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
@@ -429,8 +511,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                                A[fa], bk[cb][fb], acc[c][fa][fb], 0, 0, 0);
+                            acc[c][fa][fb] = mf_mma(A[fa], bk[cb][fb], acc[c][fa][fb]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 constexpr int kl = kStage / 32 - 1, lb = kl & 1;
@@ -448,8 +529,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
                 for (int fa = 0; fa < 2; ++fa)
-                    acc[c][fa][fb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                        dA[fa], dB[fb], acc[c][fa][fb], 0, 0, 0);
+                    acc[c][fa][fb] = mf_mma(dA[fa], dB[fb], acc[c][fa][fb]);
         }
 
         // ---- epilogue: lane holds column (k, cc) and rows (j, ab) with
@@ -472,8 +552,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #define EPI_MF_COPY(T, FA, FB)                      \
     case T:                                          \
         for (int e = 0; e < 16; ++e) {               \
-            v[0][e] = acc[0][FA][FB][e];             \
-            v[1][e] = acc[1][FA][FB][e];             \
+            v[0][e] = (int)acc[0][FA][FB][e];        \
+            v[1][e] = (int)acc[1][FA][FB][e];        \
         }                                            \
         break;
                 EPI_MF_COPY(0, 0, 0)
@@ -586,31 +666,38 @@ size_t mfma_lds_bytes(bool table_lds, uint32_t table_len)
 
 typedef void (*MfFn)(const ScanParams);
 
-static MfFn mf_pick(bool tl, bool dump)
+template <bool FP4>
+static MfFn mf_pick2(bool tl, bool dump)
 {
     if (tl)
-        return dump ? mfma_scan_kernel<true, true> : mfma_scan_kernel<true, false>;
-    return dump ? mfma_scan_kernel<false, true> : mfma_scan_kernel<false, false>;
+        return dump ? mfma_scan_kernel<true, true, FP4> : mfma_scan_kernel<true, false, FP4>;
+    return dump ? mfma_scan_kernel<false, true, FP4> : mfma_scan_kernel<false, false, FP4>;
 }
 
-hipError_t mfma_occupancy(bool table_lds, bool dump, size_t lds_bytes, int *blocks_per_cu)
+static MfFn mf_pick(bool fp4, bool tl, bool dump)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        (const void *)mf_pick(table_lds, dump),
-                                                        kMfWaves * 64, lds_bytes);
+    return fp4 ? mf_pick2<true>(tl, dump) : mf_pick2<false>(tl, dump);
 }
 
-hipError_t mfma_allow_lds(bool table_lds, bool dump, size_t lds_bytes)
+hipError_t mfma_occupancy(bool fp4, bool table_lds, bool dump, size_t lds_bytes,
+                          int *blocks_per_cu)
 {
-    return hipFuncSetAttribute((const void *)mf_pick(table_lds, dump),
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, (const void *)mf_pick(fp4, table_lds, dump), kMfWaves * 64, lds_bytes);
+}
+
+hipError_t mfma_allow_lds(bool fp4, bool table_lds, bool dump, size_t lds_bytes)
+{
+    return hipFuncSetAttribute((const void *)mf_pick(fp4, table_lds, dump),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-hipError_t launch_mfma_scan(bool table_lds, bool dump, const ScanParams &p, uint32_t grid,
-                            hipStream_t s)
+hipError_t launch_mfma_scan(bool fp4, bool table_lds, bool dump, const ScanParams &p,
+                            uint32_t grid, hipStream_t s)
 {
     const size_t lds = mfma_lds_bytes(table_lds, p.table_len);
-    hipLaunchKernelGGL(mf_pick(table_lds, dump), dim3(grid), dim3(kMfWaves * 64), lds, s, p);
+    hipLaunchKernelGGL(mf_pick(fp4, table_lds, dump), dim3(grid), dim3(kMfWaves * 64), lds, s,
+                       p);
     return hipGetLastError();
 }
 

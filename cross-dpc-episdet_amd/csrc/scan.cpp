@@ -82,9 +82,10 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
     }
 }
 
-// Kernel and tile shape: variant 2 in `tile` selects the MFMA kernel; a
-// tile naming the VALU variant or a VALU shape selects the VALU kernel;
-// otherwise the context's tile table (per sample-count bucket) decides.
+// Kernel and tile shape: variant 2 or 3 in `tile` selects that MFMA kernel
+// (int8 or FP4); a tile naming the VALU variant or a VALU shape selects the
+// VALU kernel; otherwise the context's tile table (per sample-count bucket)
+// decides.
 // A VALU shape not given by the tile or the table follows the ln(n!) table
 // placement: in LDS, 4 j-SNPs per wavefront fit in 128 VGPRs; a large table
 // in LDS allows one workgroup per CU, so it gets 16 wavefronts; with the
@@ -92,12 +93,14 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
 {
     ScanConfig mfma;
-    mfma.variant = 2;  // int8 MFMA: 64-SNP j- and k-blocks, 4 wavefronts
+    mfma.variant = 2;  // MFMA: 64-SNP j- and k-blocks, 4 wavefronts
     mfma.snps_j = 64;
     mfma.waves = 4;
     mfma.words = 1;
-    if (tile && tile->variant == 2)
+    if (tile && (tile->variant == 2 || tile->variant == 3)) {
+        mfma.variant = (int)tile->variant;
         return mfma;
+    }
     // a tile that names a VALU shape (or the VALU variant) asks for VALU
     const bool valu = tile && (tile->variant == 1 || tile->snps_j || tile->waves);
     ScanConfig cfg;
@@ -107,9 +110,11 @@ ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_s
     cfg.waves = 8;
     ScanConfig tuned;
     const bool have = tune_lookup(ctx, n_samples, tuned);
-    if (have && tuned.variant == 2 && !valu)
+    if (have && tuned.variant >= 2 && !valu) {
+        mfma.variant = tuned.variant;
         return mfma;
-    if (have && tuned.variant != 2) {
+    }
+    if (have && tuned.variant == 1) {
         cfg = tuned;
     } else {
         // no tuned VALU shape: the larger workgroup when the ln(n!) table
@@ -171,8 +176,8 @@ int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg)
     if (job.items >= 0x7FFFFFFFull)
         return EPI_EINVAL;
     job.nw = 1;
-    if (cfg.variant == 2)
-        EPI_TRY(ensure_expanded(ctx, co));
+    if (cfg.variant >= 2)
+        EPI_TRY(ensure_expanded(ctx, co, cfg.variant));
     EPI_TRY(job.list.reserve(sizeof(Cand) * kTop));
     EPI_HIP(launch_fill_empty((Cand *)job.list.ptr, kTop, ctx->stream));
     EPI_TRY(job.drows.reserve(sizeof(Row) * std::max<size_t>(job.rows.size(), 1)));
@@ -180,18 +185,26 @@ int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg)
     return EPI_OK;
 }
 
-int ensure_expanded(epi_ctx *ctx, epi_cohort *co)
+int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant)
 {
+    const bool fp4 = variant == 3;
     for (int c = 0; c < 2; ++c) {
-        if (co->E[c].ptr)
+        DevBuf &buf = fp4 ? co->F[c] : co->E[c];
+        if (buf.ptr)
             continue;
-        const uint32_t EL = (co->W[c] + 1) / 2 * 128;  // whole 128-sample stages
-        const size_t bytes = (size_t)co->Mp * 2 * std::max<uint32_t>(EL, 128);
-        EPI_TRY(co->E[c].reserve(bytes));
-        co->EL[c] = EL;
-        if (EL)
-            EPI_HIP(launch_expand((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], EL,
-                                  (uint8_t *)co->E[c].ptr, ctx->stream));
+        // rows of whole 128-byte stages: 128 samples as bytes, 256 as nibbles
+        const uint32_t L = fp4 ? (co->W[c] + 3) / 4 * 128 : (co->W[c] + 1) / 2 * 128;
+        const size_t bytes = (size_t)co->Mp * 2 * std::max<uint32_t>(L, 128);
+        EPI_TRY(buf.reserve(bytes));
+        (fp4 ? co->FL : co->EL)[c] = L;
+        if (L) {
+            if (fp4)
+                EPI_HIP(launch_expand_fp4((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L,
+                                          (uint8_t *)buf.ptr, ctx->stream));
+            else
+                EPI_HIP(launch_expand((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L,
+                                      (uint8_t *)buf.ptr, ctx->stream));
+        }
         co->bytes += bytes;
     }
     return EPI_OK;
@@ -470,7 +483,7 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
     if (!result)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
-    if (tile && tile->variant > 2)
+    if (tile && tile->variant > 3)
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
@@ -499,7 +512,7 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     if (!results)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, cohorts, n));
-    if (tile && tile->variant > 2)
+    if (tile && tile->variant > 3)
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
@@ -573,7 +586,7 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
     if (!scores)
         return EPI_EINVAL;
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
-    if (cohort->M > 512 || (tile && tile->variant > 2))
+    if (cohort->M > 512 || (tile && tile->variant > 3))
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));

@@ -89,9 +89,10 @@ extern "C" int epi_tune_set(epi_ctx *ctx, const uint32_t *max_samples, const epi
     for (uint32_t e = 0; e < n; ++e) {
         TuneEntry x;
         x.max_samples = max_samples[e];
-        x.cfg.variant = tiles[e].variant == 2 ? 2 : 1;
-        x.cfg.snps_j = x.cfg.variant == 2 ? 64 : (int)tiles[e].snps_j;
-        x.cfg.waves = x.cfg.variant == 2 ? 4 : (int)tiles[e].waves;
+        const bool mf = tiles[e].variant == 2 || tiles[e].variant == 3;
+        x.cfg.variant = mf ? (int)tiles[e].variant : 1;
+        x.cfg.snps_j = mf ? 64 : (int)tiles[e].snps_j;
+        x.cfg.waves = mf ? 4 : (int)tiles[e].waves;
         x.cfg.words = 1;
         if (!scan_config_supported(x.cfg) || (e && max_samples[e] <= max_samples[e - 1]))
             return EPI_EINVAL;
@@ -144,10 +145,10 @@ extern "C" int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, e
             t.words = 1;
             shapes.push_back(t);
         }
-    {
+    for (uint32_t v : {2u, 3u}) {
         epi_tile t;
         std::memset(&t, 0, sizeof(t));
-        t.variant = 2;
+        t.variant = v;
         t.snps_j = 64;
         t.waves = 4;
         t.words = 1;

@@ -112,7 +112,7 @@ int main(int argc, char **argv)
             mode = 2;
         else if (opt == "--cases" && !from_file && need(cases))
             cases_given = true;
-        else if (opt == "--variant" && need(v) && v <= 2)
+        else if (opt == "--variant" && need(v) && v <= 3)
             tile.variant = (uint32_t)v;
         else if (opt == "--snps-j" && need(v))
             tile.snps_j = (uint32_t)v;

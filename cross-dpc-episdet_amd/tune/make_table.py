@@ -1,7 +1,7 @@
 """Measures the scan tile table for the local GPU and writes tune/<arch>.txt.
 
 For each sample count in --samples, epi_autotune times every VALU tile
-shape and the int8 MFMA kernel on a synthetic HWE cohort of --snps SNPs and
+shape and the int8 and FP4 MFMA kernels on a synthetic HWE cohort of --snps SNPs and
 returns the fastest; with --report every shape's kernel time is printed as
 well (one JSON line per sample count).  The table maps a cohort's sample
 count to the tile of the nearest measured count (bucket edges at the
@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 import episdet as epi  # noqa: E402
 
 VALU_SHAPES = [{"variant": 1, "snps_j": j, "waves": w} for j in (1, 2, 4, 8) for w in (4, 8, 16)]
-MFMA_SHAPE = {"variant": 2}
+MFMA_SHAPES = [{"variant": 2}, {"variant": 3}]
 
 
 def shape_times(ctx, n_samples, n_snps, reps=3):
@@ -29,7 +29,7 @@ def shape_times(ctx, n_samples, n_snps, reps=3):
     out = []
     with ctx.cohort(g, p) as co:
         ref = None
-        for t in VALU_SHAPES + [MFMA_SHAPE]:
+        for t in VALU_SHAPES + MFMA_SHAPES:
             best = math.inf
             for _ in range(reps):
                 r = ctx.scan(co, tile=t)
@@ -69,7 +69,7 @@ def main():
     with open(out, "w") as f:
         f.write(f"# scan tile table for {a.arch} ({device}), measured by tune/make_table.py\n")
         f.write(f"# on HWE cohorts of {a.snps} SNPs at sample counts {' '.join(map(str, ns))}\n")
-        f.write("# max_samples snps_j waves variant (1 = VALU, 2 = int8 MFMA)\n")
+        f.write("# max_samples snps_j waves variant (1 = VALU, 2 = int8 MFMA, 3 = FP4 MFMA)\n")
         for e, (n, tile) in enumerate(rows):
             edge = int(math.sqrt(n * ns[e + 1])) if e + 1 < len(ns) else 4294967295
             f.write(f"{edge} {tile['snps_j']} {tile['waves']} {tile['variant']}\n")

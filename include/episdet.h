@@ -86,7 +86,7 @@ typedef struct {
  * (autotuned table).  Replaces the reference's block_size argument
  * (GPU/v4.cpp:212, nd_range at :277-278). */
 typedef struct {
-    uint32_t variant;   /* 0 auto, 1 VALU AND+popcount, 2 int8 MFMA */
+    uint32_t variant;   /* 0 auto, 1 VALU AND+popcount, 2 int8 MFMA, 3 FP4 MFMA */
     uint32_t snps_j;    /* j SNPs per wavefront (1, 2, 4 or 8) */
     uint32_t waves;     /* wavefronts per workgroup (1..16) */
     uint32_t words;     /* 64-bit sample words per loop step (1 or 2) */

@@ -74,7 +74,7 @@ def test_generator_argument_checks(epi):
                                   ["10", "20", "0"], ["0", "20", "32"], ["10", "20", "32", "--bogus"],
                                   ["10", "20", "32", "--cases", "11"],
                                   ["10", "20", "32", "--top", "65"],
-                                  ["10", "20", "32", "--variant", "3"],
+                                  ["10", "20", "32", "--variant", "4"],
                                   ["--bfile"], ["--bfile", "x", "--hwe", "1"],
                                   ["--bfile", "x", "--cases", "3"],
                                   ["--raw", "g", "p"], ["--raw", "g", "p", "0"],

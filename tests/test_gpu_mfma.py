@@ -1,11 +1,12 @@
-"""The int8 MFMA formulation of the 8 popcounted cells (tile variant 2):
-every count and FP32 score bit-exact against the oracle, the reference's
-known answers, and the same top-64 as the VALU kernel."""
+"""The matrix-core formulations of the 8 popcounted cells, int8 (tile
+variant 2) and FP4 (variant 3): every count and FP32 score bit-exact against
+the oracle, the reference's known answers, and the same top-64 as the VALU
+kernel."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
-MFMA = {"variant": 2}
+VARIANTS = pytest.mark.parametrize("var", [2, 3], ids=["int8", "fp4"])
 
 
 def say(*a):
@@ -15,11 +16,12 @@ def say(*a):
 @pytest.mark.parametrize("seed,N,M,cases", [(201, 1024, 256, 512), (202, 777, 70, 300),
                                             (203, 4096, 130, 2048), (204, 65, 67, 30),
                                             (205, 40000, 24, 20000), (206, 200, 3, 100)])
-def test_mfma_all_scores_bit_exact(epi, orc, ctx, seed, N, M, cases):
-    say("mfma dump", seed, N, M)
+@VARIANTS
+def test_mfma_all_scores_bit_exact(epi, orc, ctx, seed, N, M, cases, var):
+    say("mfma dump", var, seed, N, M)
     g, p = epi.generate_hwe(seed, N, M, cases)
     with ctx.cohort(g, p) as co:
-        scores, _ = ctx.scan_dump(co, tile=MFMA)
+        scores, _ = ctx.scan_dump(co, tile={"variant": var})
     ref = orc.PackedCohort(g, p).all_scores()
     bad = np.nonzero(scores.view(np.uint32) != ref.view(np.uint32))[0]
     assert bad.size == 0, (bad[:10], scores[bad[:10]], ref[bad[:10]])
@@ -27,14 +29,15 @@ def test_mfma_all_scores_bit_exact(epi, orc, ctx, seed, N, M, cases):
 
 @pytest.mark.parametrize("N,M,p_case", [(1, 5, 0.5), (63, 40, 0.5), (64, 33, 0.0), (65, 34, 1.0),
                                         (129, 66, 0.3), (500, 65, 0.6)])
-def test_mfma_all_counts_bit_exact(epi, orc, ctx, N, M, p_case):
-    say("mfma counts", N, M, p_case)
+@VARIANTS
+def test_mfma_all_counts_bit_exact(epi, orc, ctx, N, M, p_case, var):
+    say("mfma counts", var, N, M, p_case)
     rng = np.random.default_rng(N * 3 + M)
     A = rng.integers(0, 3, size=(N, M), dtype=np.uint8)
     A[rng.random((N, M)) < 0.03] = 7
     B = (rng.random(N) < p_case).astype(np.uint8)
     with ctx.cohort(A, B) as co:
-        scores, counts = ctx.scan_dump(co, tile=MFMA, with_counts=True)
+        scores, counts = ctx.scan_dump(co, tile={"variant": var}, with_counts=True)
     pc = orc.PackedCohort(A, B)
     r = 0
     for i in range(M):
@@ -45,26 +48,29 @@ def test_mfma_all_counts_bit_exact(epi, orc, ctx, N, M, p_case):
                 r += 1
 
 
-def test_mfma_reference_known_answers(epi, ctx, golden):
+@VARIANTS
+def test_mfma_reference_known_answers(epi, ctx, golden, var):
     for c in golden("kat_reference.json")["cases"]:
-        say("mfma KAT", c["N"], c["M"])
+        say("mfma KAT", var, c["N"], c["M"])
         g, p = epi.generate_reference(c["N"], c["M"])
         with ctx.cohort(g, p) as co:
-            r = ctx.scan(co, tile=MFMA)
+            r = ctx.scan(co, tile={"variant": var})
         assert [r.i, r.j, r.k] == c["runs"][0]["best"], c
         assert "%f" % r.score == c["runs"][0]["score"], c
 
 
-def test_mfma_top64_equals_valu(epi, ctx):
+@VARIANTS
+def test_mfma_top64_equals_valu(epi, ctx, var):
     for seed, N, M in [(7, 4096, 300), (8, 1500, 500), (9, 16384, 160)]:
         g, p = epi.generate_hwe(seed, N, M, N // 2)
         with ctx.cohort(g, p) as co:
             a = ctx.scan(co, tile={"variant": 1})
-            b = ctx.scan(co, tile=MFMA)
+            b = ctx.scan(co, tile={"variant": var})
         assert [(c.i, c.j, c.k, c.score) for c in a.top] == [(c.i, c.j, c.k, c.score) for c in b.top]
 
 
-def test_mfma_batch_equals_separate(epi, ctx):
+@VARIANTS
+def test_mfma_batch_equals_separate(epi, ctx, var):
     rng = np.random.default_rng(31)
     cos = []
     try:
@@ -73,7 +79,7 @@ def test_mfma_batch_equals_separate(epi, ctx):
             N = int(rng.integers(100, 5000))
             g, p = epi.generate_hwe(900 + b, N, M, N // 3)
             cos.append(ctx.cohort(g, p))
-        batch = ctx.scan_batch(cos, tile=MFMA)
+        batch = ctx.scan_batch(cos, tile={"variant": var})
         for co, rb in zip(cos, batch):
             rs = ctx.scan(co, tile={"variant": 1})
             assert [(c.i, c.j, c.k, c.score) for c in rb.top] == [(c.i, c.j, c.k, c.score) for c in rs.top]

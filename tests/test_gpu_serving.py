@@ -137,7 +137,7 @@ def test_autotune_and_tile_table(epi):
         tile, ms = ctx.autotune(4096, 400)
         say("autotune 4096 x 400:", tile, ms)
         assert ms > 0
-        if tile["variant"] == 2:
+        if tile["variant"] in (2, 3):
             assert tile["snps_j"] == 64 and tile["waves"] == 4
         else:
             assert tile["variant"] == 1
