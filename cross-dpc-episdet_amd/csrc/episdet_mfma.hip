@@ -165,8 +165,10 @@ hipError_t launch_expand_fp4(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t 
 
 // One MFMA step of a 32 x 32 tile: 16 operand bytes per lane, as 16 int8
 // samples (int32 accumulators) or 32 e2m1 samples (FP32 accumulators; the
-// instruction reads 4 of the 8 operand registers for e2m1, format code 4,
-// and scale exponent 127 = 2^0).
+// instruction reads 4 of the 8 operand registers for e2m1, format code 4).
+// The e2m1 form scales A by 2^0 and B by 2^2 (E8M0 exponents 127 and 129),
+// so its sums are 4 x the counts: the epilogue's table addresses (below
+// 2^24 as well, so still exact).
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
@@ -178,7 +180,7 @@ __device__ __forceinline__ v16f mf_mma(v4i a, v4i b, v16f c)
 {
     const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
     const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
-    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 129);
 }
 template <bool FP4>
 struct MfAcc {
@@ -234,25 +236,52 @@ struct MfLds {
     };
 };
 
-__host__ __device__ constexpr size_t mf_lds_fixed() { return (sizeof(MfLds) + 15) & ~(size_t)15; }
+// The kernel's LDS is one static block.  With the ln(n!) table in LDS the
+// table comes first, at LDS address 0, so that 4 x a count is the address
+// of its entry and the table reads need no address arithmetic; it holds
+// up to kMfTabMax entries (cohorts of up to kMfTabMax - 2 samples), the
+// rest of the 160 KiB after MfLds.
+constexpr size_t kMfLdsBudget = 160 * 1024;
+constexpr uint32_t kMfTabMax = (uint32_t)((kMfLdsBudget - sizeof(MfLds)) / 16 * 4);
+template <bool TLDS>
+struct MfShared {
+    float tab[kMfTabMax];
+    MfLds L;
+};
+template <>
+struct MfShared<false> {
+    MfLds L;
+};
+static_assert(sizeof(MfShared<true>) <= kMfLdsBudget, "LDS budget");
 
-// 16 bytes of LDS as an MFMA operand
-__device__ __forceinline__ v4i lds16(const uint8_t *p) { return *(const v4i *)p; }
+// ln(n!) of the count whose 4 x value is `off` (a byte offset into the table)
+__device__ __forceinline__ float tab_at(const unsigned char *tab, uint32_t off)
+{
+    return *reinterpret_cast<const float *>(tab + off);
+}
 
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) v4i lds_v4i;
+
+// 16 bytes of LDS at byte address `addr` as an MFMA operand
+__device__ __forceinline__ v4i lds16a(uint32_t addr)
+{
+    return *reinterpret_cast<const lds_v4i *>((uintptr_t)addr);
+}
 
 // One 16-byte-per-lane copy from global memory into LDS (lane L writes
-// bytes 16L..16L+15 after the wave-uniform LDS address `lds`).  Written as
-// asm so that the compiler's wait-count model, which counts these copies
-// as LDS operations too and then waits for every outstanding LDS read,
-// keeps exact counts for the operand reads; completion is awaited
-// explicitly (s_waitcnt vmcnt) before the barrier that publishes a stage.
-// The instruction has no register result.
-__device__ __forceinline__ void lds_copy16(const uint8_t *src, uint32_t lds)
+// bytes 16L..16L+15 after the wave-uniform LDS address `lds`), from the
+// wave-uniform base `base` (SGPR pair) plus this lane's 32-bit byte offset
+// `off`.  Written as asm so that the compiler's wait-count model, which
+// counts these copies as LDS operations too and then waits for every
+// outstanding LDS read, keeps exact counts for the operand reads;
+// completion is awaited explicitly (s_waitcnt vmcnt) before the barrier
+// that publishes a stage.  The instruction has no register result.
+__device__ __forceinline__ void lds_copy16(const uint8_t *base, uint32_t off, uint32_t lds)
 {
-    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+    asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1"
                  :
-                 : "v"(src), "s"(lds)
+                 : "v"(off), "s"(base), "s"(lds)
                  : "memory", "m0");
 }
 
@@ -262,52 +291,110 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p)
     return (uint32_t)(uintptr_t)(const lds_u8 *)p;
 }
 
-// Copies one stage from global memory straight into an LDS stage buffer
+// One stage is copied from global memory straight into an LDS stage buffer
 // (global_load_lds_dwordx4: lane L of a copy writes bytes 16L..16L+15 of a
 // 1 KiB block given by M0).  The j and k rows (32 KiB) are 32 copies of 8
-// rows each, 8 per wavefront; lane L takes row L/8 of its copy and reads
-// the global chunk that the swizzle puts in its slot L%8.  SNP
-// i's 256 bytes are one 4-lane copy per wavefront.  Every wavefront issues
-// kStageCopies + 1 copies, so one vmcnt value covers a stage on all.
-// `stage` is the LDS byte address of the MfStage to fill (wave-uniform).
-__device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *E, uint32_t Lc,
-                                               uint32_t s0, uint32_t J0, uint32_t K0, uint32_t i,
-                                               uint32_t wave, uint32_t lane)
+// rows each, 8 per wavefront (wavefronts 0, 1: j rows; 2, 3: k rows); lane L
+// takes row L/8 of its copy and reads the global chunk that the swizzle
+// puts in its slot L%8.  Row placement and swizzle key of SNP s (local to
+// its block) and plane p:
+//   j rows: row 2s + p,            key s & 7
+//   k rows: row 2s + (p ^ (s & 1)), key (s >> 1) & 7
+// (the B operand reads of one plane then alternate between the two bank
+// halves; see the operand offsets in the kernel).  SNP i's 256 bytes are one
+// 4-lane copy per wavefront.  Every wavefront issues kStageCopies + 1
+// copies, so one vmcnt value covers a stage on all.
+//
+// Per-lane source offsets of the copies within the wavefront's block (and
+// of SNP i's rows), for rows of Lc bytes; fixed for a class.
+struct MfCopyOffsets {
+    uint32_t row[kStageCopies];
+    uint32_t irow;
+};
+
+__device__ __forceinline__ MfCopyOffsets mf_copy_offsets(uint32_t Lc, uint32_t wave,
+                                                         uint32_t lane)
 {
+    MfCopyOffsets o;
     const uint32_t rsub = lane >> 3;
 #pragma unroll
     for (int q = 0; q < kStageCopies; ++q) {
-        // rows first..first+7 of the stage's 256 (j rows, then k rows)
         const uint32_t first = (wave * kStageCopies + q) * 8;
         const uint32_t blk = first >> 7, rowi = (first & 127u) + rsub;
-        const uint32_t chunk = (lane & 7u) ^ ((rowi >> 1) & 7u);
-        const uint32_t snp = (blk ? K0 : J0) + (rowi >> 1);
-        const uint8_t *src = E + ((size_t)snp * 2 + (rowi & 1u)) * Lc + s0 + 16 * chunk;
-        lds_copy16(src, stage + first * kRowBytes);  // krows follows jrows
+        const uint32_t sl = rowi >> 1;  // SNP within its block
+        const uint32_t key = blk ? (sl >> 1) & 7u : sl & 7u;
+        const uint32_t plane = (rowi & 1u) ^ (blk & sl & 1u);
+        const uint32_t chunk = (lane & 7u) ^ key;
+        o.row[q] = (2 * sl + plane) * Lc + 16 * chunk;
     }
-    if (lane < 4) {
-        const uint32_t e = 4 * wave + lane;  // 16-byte chunk of SNP i's two rows
-        const uint8_t *src = E + ((size_t)i * 2 + (e >> 3)) * Lc + s0 + 16 * (e & 7u);
-        lds_copy16(src, stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
+    const uint32_t e = 4 * wave + (lane & 3u);  // 16-byte chunk of SNP i's two rows
+    o.irow = (e >> 3) * Lc + 16 * (e & 7u);
+    return o;
+}
+
+// Copy q of the wavefront's block rows of a stage (src: the block's rows at
+// the stage's byte offset), and the 4-lane copy of SNP i's rows.
+__device__ __forceinline__ void mf_issue_copy(uint32_t stage, const uint8_t *src,
+                                              const MfCopyOffsets &o, uint32_t wave, int q)
+{
+    const uint32_t first = (wave * kStageCopies + q) * 8;
+    lds_copy16(src, o.row[q], stage + first * kRowBytes);  // krows follows jrows
+}
+__device__ __forceinline__ void mf_issue_icopy(uint32_t stage, const uint8_t *src,
+                                               const MfCopyOffsets &o, uint32_t wave,
+                                               uint32_t lane)
+{
+    if (lane < 4)
+        lds_copy16(src, o.irow, stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
+}
+
+// Issues the copies of the stage at byte s0 of the rows.  `blk_base` is the
+// wavefront's block (j-block for wavefronts 0, 1, k-block for 2, 3) and
+// `i_base` SNP i's first row; `stage` is the LDS byte address of the
+// MfStage to fill (all wave-uniform).
+__device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *blk_base,
+                                               const uint8_t *i_base, uint32_t s0,
+                                               const MfCopyOffsets &o, uint32_t wave,
+                                               uint32_t lane)
+{
+    const uint8_t *b = blk_base + s0;
+#pragma unroll
+    for (int q = 0; q < kStageCopies; ++q) {
+        const uint32_t first = (wave * kStageCopies + q) * 8;
+        lds_copy16(b, o.row[q], stage + first * kRowBytes);  // krows follows jrows
     }
+    if (lane < 4)
+        lds_copy16(i_base + s0, o.irow,
+                   stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
 }
 
 // Stages one j half of the item's (j, k) pair margins, and with half 0 also
-// the (i, j) and (i, k) margins and the single-SNP counts of both blocks.
+// the (i, j) and (i, k) margins and the single-SNP counts of both blocks,
+// all multiplied by 4 (the epilogue works on 4 x counts, the byte offsets
+// of their ln(n!) entries).
 // Entries with j >= k or i >= j belong to lanes that are scored but never
 // offered; all indices stay below Mp.
+__device__ __forceinline__ uint4 x4(uint4 v)
+{
+    return make_uint4(v.x << 2, v.y << 2, v.z << 2, v.w << 2);
+}
+
 __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST CohortDev *C,
                                                  uint32_t i, uint32_t J0, uint32_t K0,
                                                  uint32_t Mp, int fa, uint32_t tid)
 {
+    // thread tid (< 256) loads (class q / 8, row 4 (q % 8) + tid / 64,
+    // k_local tid % 64) for q < 16; the block's base is wave-uniform and
+    // the lane's offset (below 64 Mp entries) fits 32 bits
     constexpr int kLoads = 2 * 32 * 64 / (kMfWaves * 64);  // uint4 per thread
     uint4 v[kLoads];
+    const uint32_t row0 = tid >> 6, kl = tid & 63u;
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
-        const uint32_t idx = tid + 256u * q;  // (c, row, k_local)
-        const uint32_t c = idx >> 11, row = (idx >> 6) & 31u, kl = idx & 63u;
+        const uint4 *base = C->pair[q >> 3] + ((size_t)J0 * Mp + K0);
+        const uint32_t row = 4 * (q & 7) + row0;
         const uint32_t jl = 16 * (row >> 3) + 8 * fa + (row & 7u);
-        v[q] = C->pair[c][(size_t)(J0 + jl) * Mp + K0 + kl];
+        v[q] = base[jl * Mp + kl];
     }
     uint4 x = make_uint4(0, 0, 0, 0);
     uint2 y = make_uint2(0, 0);
@@ -320,18 +407,16 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
         y = C->single[c][((tid & 64u) ? K0 : J0) + e];
     }
 #pragma unroll
-    for (int q = 0; q < kLoads; ++q) {
-        const uint32_t idx = tid + 256u * q;
-        G.jk[idx >> 11][(idx >> 6) & 31u][idx & 63u] = v[q];
-    }
+    for (int q = 0; q < kLoads; ++q)
+        G.jk[q >> 3][4 * (q & 7) + row0][kl] = x4(v[q]);
     if (fa == 0) {
         const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
         if (tid & 64u) {
-            G.ik[c][e] = x;
-            G.sk[c][e] = y;
+            G.ik[c][e] = x4(x);
+            G.sk[c][e] = make_uint2(y.x << 2, y.y << 2);
         } else {
-            G.ij[c][e] = x;
-            G.sj[c][e] = y;
+            G.ij[c][e] = x4(x);
+            G.sj[c][e] = make_uint2(y.x << 2, y.y << 2);
         }
     }
 }
@@ -339,10 +424,12 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
 template <bool TLDS, bool DUMP, bool FP4>
 __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanParams P)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    MfLds &L = *reinterpret_cast<MfLds *>(lds_raw);
+    __shared__ MfShared<TLDS> sh;
+    MfLds &L = sh.L;
     const uint32_t stage_lds[2] = {lds_addr(&L.stage[0]), lds_addr(&L.stage[1])};
-    float *Tl = reinterpret_cast<float *>(lds_raw + mf_lds_fixed());
+    float *Tl = TLDS ? reinterpret_cast<float *>(&sh) : nullptr;  // the table (LDS address 0)
+    const unsigned char *tab = TLDS ? reinterpret_cast<const unsigned char *>(&sh)
+                                    : reinterpret_cast<const unsigned char *>(P.table);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -367,30 +454,40 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     const uint32_t half = lane >> 5;   // which 16 bytes of a 32-sample step
     const uint32_t l32 = lane & 31u;   // A row / B column within a tile
 
-    // lane-constant LDS offsets of the MFMA operands (bytes, within a stage
-    // buffer): row offsets, and per 32-sample step the swizzled slot
+    // Lane-constant LDS offsets of the MFMA operands, in bytes from the
+    // start of a stage buffer, per 32-byte step kk (the swizzled slot):
+    //   A (j rows): row l32 of tile fa = (j_local, a, b), j_local =
+    //     16w + 8fa + l32/4, at row 2 j_local + b; tile fa = 1 is 2048 bytes on.
+    //   B (k rows): column l32 of tile fb = 2 kq + cc is k_local = 32 kq +
+    //     l32, plane cc, at row 2 k_local + (cc ^ (k_local & 1)), so a lane
+    //     holds both planes of its k, in tiles 2 kq and 2 kq + 1; kq = 1 is
+    //     8192 bytes on.
+    //   swizzle keys: A row j_local & 7 = (l32 / 4) & 7; B row (k_local >> 1)
+    //     & 7 = (l32 / 2) & 7.  16 lanes of one operand read then hit 8 rows
+    //     in each bank half (B: alternating parity), at 8 different slots:
+    //     no bank conflicts.
+    // A stage's reads add the stage buffer's LDS address (wave-uniform) to
+    // these offsets; the fa, kq and kk parts that are constant fold into the
+    // instructions' offset fields.
     const uint32_t a_lane = (l32 >> 1) & 1u;  // g_i plane of this lane's A rows
-    uint32_t ja_off[2];
-#pragma unroll
-    for (int fa = 0; fa < 2; ++fa) {
-        // A row l32 of tile fa = (j_local, a, b), j_local = 16w + 8fa + l32/4
-        const uint32_t jl = 16 * wave + 8 * fa + (l32 >> 2);
-        ja_off[fa] = (jl * 2 + (l32 & 1u)) * kRowBytes;
-    }
-    // B column l32 of tile fb = (k_local, c), k_local = 16fb + l32/2: row 32fb + l32
-    const uint32_t kb_off = l32 * kRowBytes;
-    // swizzle keys (row >> 1) & 7 of this lane's A and B rows
-    // (A row 2 j_local + b: (row >> 1) & 7 = j_local & 7 = (l32 / 4) & 7;
-    //  B row 32 fb + l32: (l32 / 2) & 7)
     const uint32_t jx = (l32 >> 2) & 7u;
     const uint32_t kx = (l32 >> 1) & 7u;
-    uint32_t ja_slot[kStage / 32], kb_slot[kStage / 32];
+    uint32_t aoff[kStage / 32], boff[2][kStage / 32];
+    {
+        const uint32_t ja = (uint32_t)__builtin_offsetof(MfStage, jrows) +
+                            ((16 * wave + (l32 >> 2)) * 2 + (l32 & 1u)) * kRowBytes;
+        const uint32_t kb0 = (uint32_t)__builtin_offsetof(MfStage, krows) +
+                             (2 * l32 + (l32 & 1u)) * kRowBytes;
+        const uint32_t kb1 = (uint32_t)__builtin_offsetof(MfStage, krows) +
+                             (2 * l32 + (1u ^ (l32 & 1u))) * kRowBytes;
 #pragma unroll
-    for (int kk = 0; kk < kStage / 32; ++kk) {
-        ja_slot[kk] = 16 * ((2 * kk + half) ^ jx);
-        kb_slot[kk] = 16 * ((2 * kk + half) ^ kx);
+        for (int kk = 0; kk < kStage / 32; ++kk) {
+            aoff[kk] = ja + 16 * ((2 * kk + half) ^ jx);
+            boff[0][kk] = kb0 + 16 * ((2 * kk + half) ^ kx);
+            boff[1][kk] = kb1 + 16 * ((2 * kk + half) ^ kx);
+        }
     }
-    const uint32_t ei_off = a_lane * kStage + 16 * half;
+    const uint32_t eoff = (uint32_t)__builtin_offsetof(MfStage, irow) + a_lane * kStage + 16 * half;
 
     // Work-queue items: the next item is claimed while this one runs.
     if (tid == 0)
@@ -460,177 +557,203 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             // Both buffers are free here: the previous stage loop ended with
             // a barrier, and the previous item's epilogue (which reuses them
             // for margins) ended with the barrier before the next claim.
-            mf_issue_stage(stage_lds[0], E, Lc, 0u, J0, K0, i, wave, lane);
+            const MfCopyOffsets co = mf_copy_offsets(Lc, wave, lane);
+            const uint8_t *blk_base = E + (size_t)((wave >> 1) ? K0 : J0) * 2 * Lc;
+            const uint8_t *i_base = E + (size_t)i * 2 * Lc;
+            mf_issue_stage(stage_lds[0], blk_base, i_base, 0u, co, wave, lane);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            // The last step's MFMAs of a stage are issued after the barrier
-            // that ends it, together with the next stage's first reads, so
-            // the matrix cores stay busy across the barrier.
-            v4i dA[2], dB[4];  // deferred step: A (masked by SNP i) and B
+            // Steps 2 and 3 of a stage run after the barrier that ends it
+            // (their operands are in registers by then): 16 MFMAs that keep
+            // the matrix cores busy while the wavefronts meet at the barrier
+            // and while the next stage's copies are issued, one between each
+            // pair of them (a copy that waits for the memory pipeline then
+            // stalls no MFMA).  Steps 0 and 1 run before the barrier, with
+            // the reads of the following step's operands issued ahead.
+            v4i dA[2][2], dB[2][4];  // the deferred steps' A (masked by SNP i) and B
             for (uint32_t st = 0; st < n_st; ++st) {
-                const MfStage &S = L.stage[st & 1];
+                const uint32_t sb = stage_lds[st & 1];
+                // The next stage's copies go to the buffer the last stage
+                // read.  In the last stage they copy this stage again (valid
+                // rows, an idle buffer) rather than branch around the copies;
+                // like every copy they land before this stage's barrier.
+                const uint32_t nsb = stage_lds[(st + 1) & 1];
+                const uint32_t ns0 = (st + 1 < n_st ? st + 1 : st) * kStage;
                 v4i ei[kStage / 32];
 #pragma unroll
                 for (int kk = 0; kk < kStage / 32; ++kk)
-                    ei[kk] = lds16(S.irow + ei_off + 32 * kk);
-                if (st + 1 < n_st)  // the next stage, into the buffer the last one read
-                    mf_issue_stage(stage_lds[(st + 1) & 1], E, Lc, (st + 1) * kStage, J0, K0, i,
-                                   wave, lane);
-                v4i aj[2][2], bk[2][4];
-#pragma unroll
-                for (int fa = 0; fa < 2; ++fa)
-                    aj[0][fa] = lds16(S.jrows + ja_off[fa] + ja_slot[0]);
-#pragma unroll
-                for (int fb = 0; fb < 4; ++fb)
-                    bk[0][fb] = lds16(S.krows + kb_off + 32 * kRowBytes * fb + kb_slot[0]);
-                if (st > 0) {  // the previous stage's last step
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int fb = 0; fb < 4; ++fb)
-#pragma unroll
-                        for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] = mf_mma(dA[fa], dB[fb], acc[c][fa][fb]);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-#pragma unroll
-                for (int kk = 0; kk < kStage / 32 - 1; ++kk) {
-                    const int cb = kk & 1, nb = cb ^ 1;
-                    // operands of the next step (the last one into dA, dB)
+                    ei[kk] = lds16a(sb + eoff + 32 * kk);
+                v4i aj[kStage / 32][2], bk[kStage / 32][4];
+                auto read_step = [&](int kk) {
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
-                        aj[nb][fa] = lds16(S.jrows + ja_off[fa] + ja_slot[kk + 1]);
+                        aj[kk][fa] = lds16a(sb + aoff[kk] + 2048 * fa);
 #pragma unroll
                     for (int fb = 0; fb < 4; ++fb)
-                        bk[nb][fb] = lds16(S.krows + kb_off + 32 * kRowBytes * fb + kb_slot[kk + 1]);
+                        bk[kk][fb] = lds16a(sb + boff[fb & 1][kk] + 8192 * (fb >> 1));
+                };
+                read_step(0);
+                if (st > 0) {
+                    // the previous stage's steps 2, 3 with the copies of the next stage
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        if (d == 1)
+                            read_step(1);
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            __builtin_amdgcn_sched_barrier(0);
+                            acc[c][m & 1][m >> 1] =
+                                mf_mma(dA[d][m & 1], dB[d][m >> 1], acc[c][m & 1][m >> 1]);
+                            __builtin_amdgcn_sched_barrier(0);
+                            const int q = 4 * d + (m >> 1);  // copy after every second MFMA
+                            if (m & 1)
+                                mf_issue_copy(nsb, blk_base + ns0, co, wave, q);
+                            if (d == 0 && m == 0)
+                                mf_issue_icopy(nsb, i_base + ns0, co, wave, lane);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < kStageCopies; ++q)
+                        mf_issue_copy(nsb, blk_base + ns0, co, wave, q);
+                    mf_issue_icopy(nsb, i_base + ns0, co, wave, lane);
+                    read_step(1);
+                }
+                // steps 0 and 1; the operands of steps 2 and 3 are read ahead
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+                    read_step(kk + 2);
                     __builtin_amdgcn_sched_barrier(0);
                     v4i A[2];
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
-                        A[fa] = aj[cb][fa] & ei[kk];
+                        A[fa] = aj[kk][fa] & ei[kk];
 #pragma unroll
                     for (int fb = 0; fb < 4; ++fb)
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] = mf_mma(A[fa], bk[cb][fb], acc[c][fa][fb]);
+                            acc[c][fa][fb] = mf_mma(A[fa], bk[kk][fb], acc[c][fa][fb]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                constexpr int kl = kStage / 32 - 1, lb = kl & 1;
 #pragma unroll
-                for (int fa = 0; fa < 2; ++fa)
-                    dA[fa] = aj[lb][fa] & ei[kl];
+                for (int d = 0; d < 2; ++d) {
 #pragma unroll
-                for (int fb = 0; fb < 4; ++fb)
-                    dB[fb] = bk[lb][fb];
+                    for (int fa = 0; fa < 2; ++fa)
+                        dA[d][fa] = aj[d + 2][fa] & ei[d + 2];
+#pragma unroll
+                    for (int fb = 0; fb < 4; ++fb)
+                        dB[d][fb] = bk[d + 2][fb];
+                }
                 // this buffer is read and the next stage has landed in the other
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
             }
 #pragma unroll
-            for (int fb = 0; fb < 4; ++fb)
+            for (int d = 0; d < 2; ++d)
 #pragma unroll
-                for (int fa = 0; fa < 2; ++fa)
-                    acc[c][fa][fb] = mf_mma(dA[fa], dB[fb], acc[c][fa][fb]);
+                for (int fb = 0; fb < 4; ++fb)
+#pragma unroll
+                    for (int fa = 0; fa < 2; ++fa)
+                        acc[c][fa][fb] = mf_mma(dA[d][fa], dB[d][fb], acc[c][fa][fb]);
         }
 
-        // ---- epilogue: lane holds column (k, cc) and rows (j, ab) with
-        // row = 8 (e/4) + 4 half + e%4, i.e. j_local = 16w + 8fa + 2 (e/4) + half,
-        // ab = e%4, for accumulator element e.  The partner lane (lane ^ 1)
-        // holds the other cc; even lanes score j-slots 0,1, odd lanes 2,3.
-        // One tile at a time (not unrolled) keeps the scoring code single.
-        const uint32_t n0 = C->n[0], n1 = C->n[1];
-        const bool odd = (lane & 1u) != 0;
+        // ---- epilogue.  Accumulator element e of tile (fa, fb = 2 kq + cc)
+        // in lane (half, l32) is the count of rows (j, ab) x column (k, cc):
+        // j_local = 16w + 8fa + 2 (e/4) + half, ab = e%4, k_local = 32 kq + l32.
+        // So each lane holds all 8 cells of 4 (j, k) pairs per (fa, kq): it
+        // scores j-slots s = 0..3 against its own k.  Counts are handled as
+        // 4 x counts (byte offsets into the ln(n!) table): the FP4 sums come
+        // out that way, the int8 ones are shifted; margins are staged x 4.
+        // One (fa, kq) at a time (not unrolled) keeps the scoring code single.
+        const uint32_t n04 = C->n[0] << 2, n14 = C->n[1] << 2;
 #pragma unroll 1
-        for (int t = 0; t < 8; ++t) {
-            if ((t & 3) == 0) {
+        for (int t = 0; t < 4; ++t) {
+            const int fa = t >> 1, kq = t & 1;
+            if (kq == 0) {
                 // the stage buffers (or the previous half) are no longer read
                 __syncthreads();
-                mf_stage_margins(L.mg, C, i, J0, K0, Mp, t >> 2, tid);
+                mf_stage_margins(L.mg, C, i, J0, K0, Mp, fa, tid);
                 __syncthreads();
             }
-            int v[2][16];
+            uint32_t v[2][2][16];  // [class][cc][e], 4 x counts
             switch (t) {
-#define EPI_MF_COPY(T, FA, FB)                      \
-    case T:                                          \
-        for (int e = 0; e < 16; ++e) {               \
-            v[0][e] = (int)acc[0][FA][FB][e];        \
-            v[1][e] = (int)acc[1][FA][FB][e];        \
-        }                                            \
+#define EPI_MF_COPY(T, FA, KQ)                                                       \
+    case T:                                                                          \
+        for (int c = 0; c < 2; ++c)                                                  \
+            for (int cc = 0; cc < 2; ++cc)                                           \
+                for (int e = 0; e < 16; ++e)                                         \
+                    v[c][cc][e] = FP4 ? (uint32_t)acc[c][FA][2 * KQ + cc][e]         \
+                                      : (uint32_t)acc[c][FA][2 * KQ + cc][e] << 2;   \
         break;
                 EPI_MF_COPY(0, 0, 0)
                 EPI_MF_COPY(1, 0, 1)
-                EPI_MF_COPY(2, 0, 2)
-                EPI_MF_COPY(3, 0, 3)
-                EPI_MF_COPY(4, 1, 0)
-                EPI_MF_COPY(5, 1, 1)
-                EPI_MF_COPY(6, 1, 2)
-                EPI_MF_COPY(7, 1, 3)
+                EPI_MF_COPY(2, 1, 0)
+                EPI_MF_COPY(3, 1, 1)
 #undef EPI_MF_COPY
             }
-            const uint32_t fa = (uint32_t)t >> 2, fb = (uint32_t)t & 3u;
-            const uint32_t k = K0 + 16 * fb + (l32 >> 1);
-            const uint32_t kl = k - K0;
-            int w[2][16];  // the partner lane's values: swap lanes 2m and 2m+1 (DPP)
+            const uint32_t kl = 32 * (uint32_t)kq + l32;
+            const uint32_t k = K0 + kl;
+            uint4 ik[2];
+            uint2 sk[2], si[2];
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
+            for (int c = 0; c < 2; ++c) {
+                ik[c] = L.mg.ik[c][kl];
+                sk[c] = L.mg.sk[c][kl];
+                const uint2 s1 = mf::ldc2(C->single[c] + i);
+                si[c] = make_uint2(s1.x << 2, s1.y << 2);
+            }
 #pragma unroll
-                for (int e = 0; e < 16; ++e)
-                    w[c][e] = __builtin_amdgcn_mov_dpp(v[c][e], 0xB1, 0xF, 0xF, false);
-#pragma unroll
-            for (int sl = 0; sl < 2; ++sl) {
-                const uint32_t slot = (odd ? 2u : 0u) + (uint32_t)sl;  // e / 4
-                const uint32_t j = J0 + 16 * wave + 8 * fa + 2 * slot + half;
+            for (int sl = 0; sl < 4; ++sl) {
+                const uint32_t jl = 16 * wave + 8 * (uint32_t)fa + 2 * sl + half;
+                const uint32_t j = J0 + jl;
                 const bool valid = (i < j) && (j < k) && (k < M);
-                const uint32_t jl = j - J0;
                 // every lane is scored (lanes outside i < j < k score +inf):
                 // no divergent branch around the batched table reads.  The
                 // pair table holds every (p, q), so even those lanes' counts
-                // are exact and every index below is at most N + 1, inside
-                // the table.
+                // are exact and every offset below is at most 4 (N + 1),
+                // inside the table.
                 uint32_t x[2][27];
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
                     uint32_t t8[8];
 #pragma unroll
                     for (int ab = 0; ab < 4; ++ab) {
-                        // element of this lane's j-slot: even lanes slots 0,1, odd 2,3
-                        const uint32_t mine = (uint32_t)(odd ? v[c][8 + 4 * sl + ab] : v[c][4 * sl + ab]);
-                        const uint32_t theirs = (uint32_t)(odd ? w[c][8 + 4 * sl + ab] : w[c][4 * sl + ab]);
-                        t8[2 * ab + 0] = odd ? theirs : mine;  // cc = 0 lives in the even lane
-                        t8[2 * ab + 1] = odd ? mine : theirs;
+                        t8[2 * ab + 0] = v[c][0][4 * sl + ab];
+                        t8[2 * ab + 1] = v[c][1][4 * sl + ab];
                     }
                     const uint4 ij = L.mg.ij[c][jl];
-                    const uint4 ik = L.mg.ik[c][kl];
-                    const uint4 jk = L.mg.jk[c][8 * wave + 2 * slot + half][kl];
-                    const uint2 si = mf::ldc2(C->single[c] + i);
+                    const uint4 jk = L.mg.jk[c][8 * wave + 2 * sl + half][kl];
                     const uint2 sj = L.mg.sj[c][jl];
-                    const uint2 sk = L.mg.sk[c][kl];
                     const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
-                    const uint32_t aik[4] = {ik.x, ik.y, ik.z, ik.w};
+                    const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
                     const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
-                    const uint32_t asi[2] = {si.x, si.y};
+                    const uint32_t asi[2] = {si[c].x, si[c].y};
                     const uint32_t asj[2] = {sj.x, sj.y};
-                    const uint32_t ask[2] = {sk.x, sk.y};
-                    derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n1 : n0, x[c]);
+                    const uint32_t ask[2] = {sk[c].x, sk[c].y};
+                    derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
                 }
                 // K2 in the reference order (GPU/v4.cpp:579-583,
-                // CPU/v2.cpp:516-519); the table reads of 9 cells are issued
-                // ahead of their ordered sums
+                // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
+                // summed over the cells in order from 0 (the first term needs
+                // no add: no term is -0, so 0 + t == t bit for bit); the table
+                // reads of 9 cells are issued ahead of their ordered sums.
                 float sc = 0.0f;
 #pragma unroll
                 for (int m0 = 0; m0 < 27; m0 += 9) {
                     float ta[9], tb[9], td[9];
 #pragma unroll
                     for (int m = 0; m < 9; ++m) {
-                        const uint32_t ib = x[0][m0 + m], id = x[1][m0 + m];
-                        const uint32_t ia = ib + id + 1;
-                        ta[m] = TLDS ? Tl[ia] : P.table[ia];
-                        tb[m] = TLDS ? Tl[ib] : P.table[ib];
-                        td[m] = TLDS ? Tl[id] : P.table[id];
+                        const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
+                        ta[m] = tab_at(tab + 4, ob + od);
+                        tb[m] = tab_at(tab, ob);
+                        td[m] = tab_at(tab, od);
                     }
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int m = 0; m < 9; ++m)
-                        sc += (ta[m] - tb[m]) - td[m];
+                    for (int m = 0; m < 9; ++m) {
+                        const float term = (ta[m] - tb[m]) - td[m];
+                        sc = (m0 + m == 0) ? term : sc + term;
+                    }
                 }
                 const float s = valid ? fabsf(sc) : INFINITY;
                 if (DUMP && valid) {
@@ -643,8 +766,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     if (P.dump_counts) {
 #pragma unroll
                         for (int m = 0; m < 27; ++m) {
-                            P.dump_counts[rk * 54 + m] = x[0][m];
-                            P.dump_counts[rk * 54 + 27 + m] = x[1][m];
+                            P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
+                            P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
                         }
                     }
                 }
@@ -659,9 +782,13 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     }
 }
 
+// Static LDS of the instance; a table too large for LDS reports more than
+// any device has, so that the caller places it in global memory.
 size_t mfma_lds_bytes(bool table_lds, uint32_t table_len)
 {
-    return mf_lds_fixed() + (table_lds ? (size_t)table_len * 4 : 0);
+    if (!table_lds)
+        return sizeof(MfShared<false>);
+    return table_len <= kMfTabMax ? sizeof(MfShared<true>) : ((size_t)1 << 40);
 }
 
 typedef void (*MfFn)(const ScanParams);
@@ -679,25 +806,27 @@ static MfFn mf_pick(bool fp4, bool tl, bool dump)
     return fp4 ? mf_pick2<true>(tl, dump) : mf_pick2<false>(tl, dump);
 }
 
+// The MFMA kernels' LDS is static (no dynamic LDS at launch).
 hipError_t mfma_occupancy(bool fp4, bool table_lds, bool dump, size_t lds_bytes,
                           int *blocks_per_cu)
 {
+    (void)lds_bytes;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, (const void *)mf_pick(fp4, table_lds, dump), kMfWaves * 64, lds_bytes);
+        blocks_per_cu, (const void *)mf_pick(fp4, table_lds, dump), kMfWaves * 64, 0);
 }
 
 hipError_t mfma_allow_lds(bool fp4, bool table_lds, bool dump, size_t lds_bytes)
 {
-    return hipFuncSetAttribute((const void *)mf_pick(fp4, table_lds, dump),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    (void)fp4, (void)table_lds, (void)dump, (void)lds_bytes;
+    return hipSuccess;
 }
 
 hipError_t launch_mfma_scan(bool fp4, bool table_lds, bool dump, const ScanParams &p,
                             uint32_t grid, hipStream_t s)
 {
-    const size_t lds = mfma_lds_bytes(table_lds, p.table_len);
-    hipLaunchKernelGGL(mf_pick(fp4, table_lds, dump), dim3(grid), dim3(kMfWaves * 64), lds, s,
-                       p);
+    if (table_lds && p.table_len > kMfTabMax)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mf_pick(fp4, table_lds, dump), dim3(grid), dim3(kMfWaves * 64), 0, s, p);
     return hipGetLastError();
 }
 

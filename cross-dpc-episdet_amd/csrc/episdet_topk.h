@@ -113,7 +113,12 @@ __device__ __forceinline__ void wave_compact(WaveTop &W, uint32_t lane)
 __device__ __forceinline__ void wave_offer(WaveTop &W, bool valid, float s, uint64_t key,
                                            uint32_t lane)
 {
-    const bool pass = valid && s <= W.gthr && less(s, key, W.ts, W.tk);
+    // cheap pre-filter on the score alone (a superset of what passes); the
+    // key comparison of ties runs only when some lane gets that far
+    const bool maybe = valid && s <= W.gthr && s <= W.ts;
+    if (__ballot(maybe) == 0)
+        return;
+    const bool pass = maybe && less(s, key, W.ts, W.tk);
     const uint64_t mask = __ballot(pass);
     if (mask == 0)
         return;
