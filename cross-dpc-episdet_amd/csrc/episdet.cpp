@@ -322,6 +322,10 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     CohortDev h = co->host_desc();
     if (st == EPI_OK && e == hipSuccess)
         e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream);
+    // the pair and single margins are part of the resident cohort: every
+    // scan reads them, none recomputes them
+    if (st == EPI_OK && e == hipSuccess)
+        e = launch_pairs((const CohortDev *)co->desc.ptr, h, ctx->stream);
     if (st == EPI_OK && e == hipSuccess)
         e = hipStreamSynchronize(ctx->stream);
     if (st == EPI_OK && e != hipSuccess)

@@ -51,16 +51,18 @@ struct HostBuf {
 };
 
 // One cohort's scan in progress: its triplet-tile rows (cohort-local item
-// numbering), how many rows have been launched, and its running top-kTop
-// candidate list on the device.
+// numbering; the device copy is the cohort's cached one), how many rows
+// have been launched, and its running top-kTop candidate list on the device
+// (a slice of the context's list pool, or `own_list` for server jobs).
 struct Job {
     epi_cohort *co = nullptr;
-    std::vector<Row> rows;  // host copy (ranges are chosen on the host)
-    DevBuf drows;           // resident copy read by the scan kernel
+    std::vector<Row> rows;        // host copy (ranges are chosen on the host)
+    const Row *drows = nullptr;   // resident copy read by the scan kernel
     uint64_t items = 0;
-    uint32_t nw = 1;        // wavefronts per item (i values per item)
+    uint32_t nw = 1;              // wavefronts per item (i values per item)
     size_t next_row = 0;
-    DevBuf list;
+    Cand *list = nullptr;
+    DevBuf own_list;
 };
 
 // Tile table entry: applies to n_samples <= max_samples.
@@ -115,8 +117,12 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
                           bool any_items, bool timed, float *dump_scores, uint32_t *dump_counts);
 int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool dump,
                 LaunchPlan &plan);
-int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg);
-int enqueue_pairs(epi_ctx *ctx, epi_cohort *co);
+// Prepares a job: the cohort's cached rows for the tile width, its expanded
+// planes for an MFMA variant, and the running list: `list` (kTop entries,
+// cleared by the caller) or, when null, a list of the job's own, cleared here.
+int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg, Cand *list);
+// Reserves the context's list pool for n jobs and clears it (one launch).
+int reserve_lists(epi_ctx *ctx, uint32_t n);
 // Builds the cohort's expanded planes for an MFMA variant (2: 0/1 bytes,
 // 3: e2m1 nibbles) if they do not exist.
 int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant);
@@ -145,6 +151,7 @@ struct epi_ctx {
     std::vector<epi::TuneEntry> tune;  // autotuned tile table
     // per-launch scratch, grown on demand
     epi::DevBuf round, fin, counters, slots, rescored;
+    epi::DevBuf lists;  // running top-kTop lists of the jobs of one call
     epi::HostBuf stage;
     hipEvent_t stage_ev = nullptr;  // marks the last copy out of `stage`
     bool stage_pending = false;
@@ -164,6 +171,11 @@ struct epi_cohort {
     uint32_t EL[2] = {0, 0};  // their row pitch in bytes
     epi::DevBuf F[2];  // e2m1 nibble planes for the FP4 MFMA variant, built on first use
     uint32_t FL[2] = {0, 0};  // their row pitch in bytes
+    // triplet-tile rows for tile width rows_jp (0: none yet), built on first use
+    std::vector<epi::Row> rows;
+    uint64_t rows_items = 0;
+    int rows_jp = 0;
+    epi::DevBuf drows;
     uint64_t bytes = 0;
     epi::CohortDev host_desc() const;
     ~epi_cohort();

@@ -164,24 +164,50 @@ int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool du
     return EPI_OK;
 }
 
-int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg)
+// The cohort's triplet-tile rows for tile width jp, built and uploaded once
+// (one i per queue item: a wavefront of the VALU kernel or a workgroup of
+// the MFMA kernel takes one item at a time).
+static int cohort_rows(epi_ctx *ctx, epi_cohort *co, int jp)
+{
+    if (co->rows_jp == jp)
+        return EPI_OK;
+    std::vector<Row> rows;
+    uint64_t items = 0;
+    build_rows(0, co->M, co->Mp, jp, 1, rows, items);
+    if (items >= 0x7FFFFFFFull)
+        return EPI_EINVAL;
+    EPI_TRY(co->drows.reserve(sizeof(Row) * std::max<size_t>(rows.size(), 1)));
+    EPI_TRY(stage_upload(ctx, co->drows.ptr, rows.data(), sizeof(Row) * rows.size()));
+    co->rows = std::move(rows);
+    co->rows_items = items;
+    co->rows_jp = jp;
+    return EPI_OK;
+}
+
+int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg, Cand *list)
 {
     job.co = co;
-    job.rows.clear();
-    job.items = 0;
     job.next_row = 0;
-    // one i per queue item: a wavefront (VALU kernel) or a workgroup (MFMA
-    // kernel) takes one item at a time
-    build_rows(0, co->M, co->Mp, cfg.snps_j, 1, job.rows, job.items);
-    if (job.items >= 0x7FFFFFFFull)
-        return EPI_EINVAL;
     job.nw = 1;
+    EPI_TRY(cohort_rows(ctx, co, cfg.snps_j));
+    job.rows = co->rows;
+    job.items = co->rows_items;
+    job.drows = (const Row *)co->drows.ptr;
     if (cfg.variant >= 2)
         EPI_TRY(ensure_expanded(ctx, co, cfg.variant));
-    EPI_TRY(job.list.reserve(sizeof(Cand) * kTop));
-    EPI_HIP(launch_fill_empty((Cand *)job.list.ptr, kTop, ctx->stream));
-    EPI_TRY(job.drows.reserve(sizeof(Row) * std::max<size_t>(job.rows.size(), 1)));
-    EPI_TRY(stage_upload(ctx, job.drows.ptr, job.rows.data(), sizeof(Row) * job.rows.size()));
+    if (!list) {
+        EPI_TRY(job.own_list.reserve(sizeof(Cand) * kTop));
+        list = (Cand *)job.own_list.ptr;
+        EPI_HIP(launch_fill_empty(list, kTop, ctx->stream));
+    }
+    job.list = list;
+    return EPI_OK;
+}
+
+int reserve_lists(epi_ctx *ctx, uint32_t n)
+{
+    EPI_TRY(ctx->lists.reserve(sizeof(Cand) * kTop * std::max<size_t>(n, 1)));
+    EPI_HIP(launch_fill_empty((Cand *)ctx->lists.ptr, (size_t)kTop * n, ctx->stream));
     return EPI_OK;
 }
 
@@ -207,13 +233,6 @@ int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant)
         }
         co->bytes += bytes;
     }
-    return EPI_OK;
-}
-
-int enqueue_pairs(epi_ctx *ctx, epi_cohort *co)
-{
-    const CohortDev h = co->host_desc();
-    EPI_HIP(launch_pairs((const CohortDev *)co->desc.ptr, h, ctx->stream));
     return EPI_OK;
 }
 
@@ -265,7 +284,7 @@ int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint3
     for (uint32_t b = 0; b < n; ++b) {
         const Job &J = *jobs[b];
         descs[b] = J.co->host_desc();
-        lists[b] = (Cand *)J.list.ptr;
+        lists[b] = J.list;
         const size_t r0 = ranges[b].first, r1 = ranges[b].second;
         if (r1 <= r0)
             continue;
@@ -273,7 +292,7 @@ int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint3
         const uint32_t first = J.rows[r0].item_start;
         const uint32_t end = last.item_start + (last.icount + J.nw - 1) / J.nw;
         Segment sg;
-        sg.rows = (const Row *)J.drows.ptr;
+        sg.rows = J.drows;
         sg.cohort = b;
         sg.row_begin = (uint32_t)r0;
         sg.row_end = (uint32_t)r1;
@@ -359,7 +378,7 @@ static int upload_rescore(epi_ctx *ctx, Job *const *jobs, uint32_t n)
     std::vector<unsigned char> host(total, 0);
     for (uint32_t b = 0; b < n; ++b) {
         ((CohortDev *)host.data())[b] = jobs[b]->co->host_desc();
-        ((Cand **)(host.data() + ld))[b] = (Cand *)jobs[b]->list.ptr;
+        ((Cand **)(host.data() + ld))[b] = jobs[b]->list;
     }
     EPI_TRY(ctx->fin.reserve(total));
     EPI_TRY(ctx->rescored.reserve(sizeof(Rescored) * (size_t)n * kTop));
@@ -493,8 +512,8 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
     EPI_TRY(plan_launch(ctx, cfg, cohort->N + 2, false, plan));
     const auto t0 = std::chrono::steady_clock::now();
     Job job;
-    EPI_TRY(job_init(ctx, job, cohort, cfg));
-    EPI_TRY(enqueue_pairs(ctx, cohort));
+    EPI_TRY(reserve_lists(ctx, 1));
+    EPI_TRY(job_init(ctx, job, cohort, cfg, (Cand *)ctx->lists.ptr));
     Job *jp[1] = {&job};
     const std::pair<size_t, size_t> all(0, job.rows.size());
     EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, true, nullptr, nullptr));
@@ -527,9 +546,9 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     std::vector<Job> jobs(n);
     std::vector<Job *> jp(n);
     std::vector<std::pair<size_t, size_t>> ranges(n);
+    EPI_TRY(reserve_lists(ctx, n));
     for (uint32_t b = 0; b < n; ++b) {
-        EPI_TRY(job_init(ctx, jobs[b], cohorts[b], cfg));
-        EPI_TRY(enqueue_pairs(ctx, cohorts[b]));
+        EPI_TRY(job_init(ctx, jobs[b], cohorts[b], cfg, (Cand *)ctx->lists.ptr + (size_t)kTop * b));
         jp[b] = &jobs[b];
         ranges[b] = std::make_pair((size_t)0, jobs[b].rows.size());
     }
@@ -603,8 +622,7 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
     if (counts)
         EPI_HIP(launch_fill_u32((uint32_t *)dc.ptr, 0xFFFFFFFFu, ntrip * 54, ctx->stream));
     Job job;
-    EPI_TRY(job_init(ctx, job, cohort, cfg));
-    EPI_TRY(enqueue_pairs(ctx, cohort));
+    EPI_TRY(job_init(ctx, job, cohort, cfg, nullptr));
     Job *jp[1] = {&job};
     const std::pair<size_t, size_t> all(0, job.rows.size());
     EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, false, (float *)ds.ptr, (uint32_t *)dc.ptr));

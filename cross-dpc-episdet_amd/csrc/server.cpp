@@ -122,9 +122,7 @@ int epi_server::admit_locked_ctx(Request *r)
     int st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
                                   &r->co);
     if (st == EPI_OK)
-        st = job_init(ctx, r->job, r->co, scfg);
-    if (st == EPI_OK)
-        st = enqueue_pairs(ctx, r->co);
+        st = job_init(ctx, r->job, r->co, scfg, nullptr);
     r->admitted = Clock::now();
     return st;
 }
@@ -136,8 +134,9 @@ void epi_server::release_locked_ctx(Request *r)
         delete r->co;
         r->co = nullptr;
     }
-    r->job.drows.release();
-    r->job.list.release();
+    r->job.drows = nullptr;
+    r->job.list = nullptr;
+    r->job.own_list.release();
     r->job.rows.clear();
     r->job.rows.shrink_to_fit();
 }
