@@ -254,6 +254,16 @@ struct MfShared<false> {
 };
 static_assert(sizeof(MfShared<true>) <= kMfLdsBudget, "LDS budget");
 
+// Rank of triplet i < j < k among all triplets of M SNPs in lexicographic
+// order (the dump layout).
+__device__ __forceinline__ uint64_t dump_rank(uint32_t M, uint32_t i, uint32_t j, uint32_t k)
+{
+    auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; };
+    auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
+    const uint64_t MM = M;
+    return c3(MM) - c3(MM - i) + c2(MM - i - 1) - c2(MM - j) + (uint64_t)(k - j - 1);
+}
+
 // ln(n!) of the count whose 4 x value is `off` (a byte offset into the table)
 __device__ __forceinline__ float tab_at(const unsigned char *tab, uint32_t off)
 {
@@ -707,70 +717,67 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 const uint32_t jl = 16 * wave + 8 * (uint32_t)fa + 2 * sl + half;
                 const uint32_t j = J0 + jl;
                 const bool valid = (i < j) && (j < k) && (k < M);
-                // every lane is scored (lanes outside i < j < k score +inf):
-                // no divergent branch around the batched table reads.  The
-                // pair table holds every (p, q), so even those lanes' counts
-                // are exact and every offset below is at most 4 (N + 1),
-                // inside the table.
-                uint32_t x[2][27];
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    uint32_t t8[8];
-#pragma unroll
-                    for (int ab = 0; ab < 4; ++ab) {
-                        t8[2 * ab + 0] = v[c][0][4 * sl + ab];
-                        t8[2 * ab + 1] = v[c][1][4 * sl + ab];
-                    }
-                    const uint4 ij = L.mg.ij[c][jl];
-                    const uint4 jk = L.mg.jk[c][8 * wave + 2 * sl + half][kl];
-                    const uint2 sj = L.mg.sj[c][jl];
-                    const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
-                    const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
-                    const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
-                    const uint32_t asi[2] = {si[c].x, si[c].y};
-                    const uint32_t asj[2] = {sj.x, sj.y};
-                    const uint32_t ask[2] = {sk[c].x, sk[c].y};
-                    derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
-                }
-                // K2 in the reference order (GPU/v4.cpp:579-583,
-                // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
-                // summed over the cells in order from 0 (the first term needs
-                // no add: no term is -0, so 0 + t == t bit for bit); the table
-                // reads of 9 cells are issued ahead of their ordered sums.
+                // Only lanes with i < j < k are scored (a slot with none is
+                // skipped).  The pair table holds every (p, q), so counts are
+                // exact for every lane and every offset below is at most
+                // 4 (N + 1), inside the table.
                 float sc = 0.0f;
+                if (valid) {
+                    uint32_t x[2][27];
 #pragma unroll
-                for (int m0 = 0; m0 < 27; m0 += 9) {
-                    float ta[9], tb[9], td[9];
+                    for (int c = 0; c < 2; ++c) {
+                        uint32_t t8[8];
 #pragma unroll
-                    for (int m = 0; m < 9; ++m) {
-                        const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
-                        ta[m] = tab_at(tab + 4, ob + od);
-                        tb[m] = tab_at(tab, ob);
-                        td[m] = tab_at(tab, od);
+                        for (int ab = 0; ab < 4; ++ab) {
+                            t8[2 * ab + 0] = v[c][0][4 * sl + ab];
+                            t8[2 * ab + 1] = v[c][1][4 * sl + ab];
+                        }
+                        const uint4 ij = L.mg.ij[c][jl];
+                        const uint4 jk = L.mg.jk[c][8 * wave + 2 * sl + half][kl];
+                        const uint2 sj = L.mg.sj[c][jl];
+                        const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
+                        const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
+                        const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
+                        const uint32_t asi[2] = {si[c].x, si[c].y};
+                        const uint32_t asj[2] = {sj.x, sj.y};
+                        const uint32_t ask[2] = {sk[c].x, sk[c].y};
+                        derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
                     }
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int m = 0; m < 9; ++m) {
-                        const float term = (ta[m] - tb[m]) - td[m];
-                        sc = (m0 + m == 0) ? term : sc + term;
-                    }
-                }
-                const float s = valid ? fabsf(sc) : INFINITY;
-                if (DUMP && valid) {
-                    const uint64_t MM = M;
-                    auto c3 = [](uint64_t n) { return n < 3 ? 0 : n * (n - 1) * (n - 2) / 6; };
-                    auto c2 = [](uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; };
-                    const uint64_t rk = c3(MM) - c3(MM - i) + c2(MM - i - 1) - c2(MM - j) +
-                                        (uint64_t)(k - j - 1);
-                    P.dump_scores[rk] = s;
-                    if (P.dump_counts) {
+                    if (DUMP && P.dump_counts) {
+                        const uint64_t rk = dump_rank(M, i, j, k);
 #pragma unroll
                         for (int m = 0; m < 27; ++m) {
                             P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
                             P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
                         }
                     }
+                    // K2 in the reference order (GPU/v4.cpp:579-583,
+                    // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
+                    // summed over the cells in order from 0 (the first term
+                    // needs no add: no term is -0, so 0 + t == t bit for bit);
+                    // the table reads of 9 cells are issued ahead of their
+                    // ordered sums.
+#pragma unroll
+                    for (int m0 = 0; m0 < 27; m0 += 9) {
+                        float ta[9], tb[9], td[9];
+#pragma unroll
+                        for (int m = 0; m < 9; ++m) {
+                            const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
+                            ta[m] = tab_at(tab + 4, ob + od);
+                            tb[m] = tab_at(tab, ob);
+                            td[m] = tab_at(tab, od);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int m = 0; m < 9; ++m) {
+                            const float term = (ta[m] - tb[m]) - td[m];
+                            sc = (m0 + m == 0) ? term : sc + term;
+                        }
+                    }
                 }
+                const float s = valid ? fabsf(sc) : INFINITY;
+                if (DUMP && valid)
+                    P.dump_scores[dump_rank(M, i, j, k)] = s;
                 if (!DUMP)
                     topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
             }
