@@ -214,13 +214,10 @@ struct MfStage {
 };
 static_assert(__builtin_offsetof(MfStage, krows) == kStageRows * kRowBytes, "k rows follow j rows");
 
-// The margins an item's epilogue reads, staged in LDS over the (then idle)
-// stage buffers.  The j-block is split in two halves of 32 SNPs, one per
-// accumulator row tile fa: half fa holds j_local = 16w + 8fa + r (w < 4,
-// r < 8) as row 8w + r.
-constexpr int kJkPitch = 64 + 1;  // uint4 per JK row (+1: bank offset)
+// The margins of SNP i and the single-SNP counts an item's epilogue reads,
+// staged in LDS over the (then idle) stage buffers, x 4.  The (j, k) pair
+// margins are read from global memory by each lane, a slot ahead.
 struct MfMargins {
-    uint4 jk[2][32][kJkPitch];  // pair margins of (j, k), one j half, both classes
     uint4 ij[2][64];            // pair margins of (i, j)
     uint4 ik[2][64];            // pair margins of (i, k)
     uint2 sj[2][64];            // single-SNP counts of the j-block
@@ -391,43 +388,20 @@ __device__ __forceinline__ uint4 x4(uint4 v)
 
 __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST CohortDev *C,
                                                  uint32_t i, uint32_t J0, uint32_t K0,
-                                                 uint32_t Mp, int fa, uint32_t tid)
+                                                 uint32_t Mp, uint32_t tid)
 {
-    // thread tid (< 256) loads (class q / 8, row 4 (q % 8) + tid / 64,
-    // k_local tid % 64) for q < 16; the block's base is wave-uniform and
-    // the lane's offset (below 64 Mp entries) fits 32 bits
-    constexpr int kLoads = 2 * 32 * 64 / (kMfWaves * 64);  // uint4 per thread
-    uint4 v[kLoads];
-    const uint32_t row0 = tid >> 6, kl = tid & 63u;
-#pragma unroll
-    for (int q = 0; q < kLoads; ++q) {
-        const uint4 *base = C->pair[q >> 3] + ((size_t)J0 * Mp + K0);
-        const uint32_t row = 4 * (q & 7) + row0;
-        const uint32_t jl = 16 * (row >> 3) + 8 * fa + (row & 7u);
-        v[q] = base[jl * Mp + kl];
-    }
-    uint4 x = make_uint4(0, 0, 0, 0);
-    uint2 y = make_uint2(0, 0);
-    if (fa == 0) {
-        const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
-        if (tid & 64u)
-            x = C->pair[c][(size_t)i * Mp + K0 + e];
-        else
-            x = C->pair[c][(size_t)i * Mp + J0 + e];
-        y = C->single[c][((tid & 64u) ? K0 : J0) + e];
-    }
-#pragma unroll
-    for (int q = 0; q < kLoads; ++q)
-        G.jk[q >> 3][4 * (q & 7) + row0][kl] = x4(v[q]);
-    if (fa == 0) {
-        const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
-        if (tid & 64u) {
-            G.ik[c][e] = x4(x);
-            G.sk[c][e] = make_uint2(y.x << 2, y.y << 2);
-        } else {
-            G.ij[c][e] = x4(x);
-            G.sj[c][e] = make_uint2(y.x << 2, y.y << 2);
-        }
+    // thread tid (< 256): class (tid / 128), block (tid / 64 % 2: j or k),
+    // SNP tid % 64 of the block
+    const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
+    const bool kb = (tid & 64u) != 0;
+    const uint4 x = C->pair[c][(size_t)i * Mp + (kb ? K0 : J0) + e];
+    const uint2 y = C->single[c][(kb ? K0 : J0) + e];
+    if (kb) {
+        G.ik[c][e] = x4(x);
+        G.sk[c][e] = make_uint2(y.x << 2, y.y << 2);
+    } else {
+        G.ij[c][e] = x4(x);
+        G.sj[c][e] = make_uint2(y.x << 2, y.y << 2);
     }
 }
 
@@ -676,15 +650,13 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         // out that way, the int8 ones are shifted; margins are staged x 4.
         // One (fa, kq) at a time (not unrolled) keeps the scoring code single.
         const uint32_t n04 = C->n[0] << 2, n14 = C->n[1] << 2;
+        // the stage buffers are no longer read
+        __syncthreads();
+        mf_stage_margins(L.mg, C, i, J0, K0, Mp, tid);
+        __syncthreads();
 #pragma unroll 1
         for (int t = 0; t < 4; ++t) {
             const int fa = t >> 1, kq = t & 1;
-            if (kq == 0) {
-                // the stage buffers (or the previous half) are no longer read
-                __syncthreads();
-                mf_stage_margins(L.mg, C, i, J0, K0, Mp, fa, tid);
-                __syncthreads();
-            }
             uint32_t v[2][2][16];  // [class][cc][e], 4 x counts
             switch (t) {
 #define EPI_MF_COPY(T, FA, KQ)                                                       \
@@ -712,8 +684,27 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 const uint2 s1 = mf::ldc2(C->single[c] + i);
                 si[c] = make_uint2(s1.x << 2, s1.y << 2);
             }
+            // (j, k) pair margins of the lane's slots, read a slot ahead:
+            // row J0 + jl of pair[c], k columns K0 + kl (32 consecutive k per
+            // half-wavefront); every (p, q) < Mp is in the table
+            const uint32_t jl0 = 16 * wave + 8 * (uint32_t)fa + half;
+            const uint4 *jkp[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                jkp[c] = C->pair[c] + ((size_t)J0 + jl0) * Mp + k;
+            uint4 jkn[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                jkn[c] = jkp[c][0];
 #pragma unroll
             for (int sl = 0; sl < 4; ++sl) {
+                uint4 jkc[2];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    jkc[c] = x4(jkn[c]);
+                    if (sl + 1 < 4)
+                        jkn[c] = jkp[c][(size_t)(2 * sl + 2) * Mp];
+                }
                 const uint32_t jl = 16 * wave + 8 * (uint32_t)fa + 2 * sl + half;
                 const uint32_t j = J0 + jl;
                 const bool valid = (i < j) && (j < k) && (k < M);
@@ -733,7 +724,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                             t8[2 * ab + 1] = v[c][1][4 * sl + ab];
                         }
                         const uint4 ij = L.mg.ij[c][jl];
-                        const uint4 jk = L.mg.jk[c][8 * wave + 2 * sl + half][kl];
+                        const uint4 jk = jkc[c];
                         const uint2 sj = L.mg.sj[c][jl];
                         const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
                         const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
