@@ -251,7 +251,7 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps)
 // samples with a missing phenotype.
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out)
+                         epi_cohort **out, hipStream_t strm)
 {
     // class split in sample order (GPU/v4.cpp:135-171): phenotype 1 = case
     std::vector<uint32_t> order;
@@ -281,7 +281,7 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
         co->W[c] = (co->n[c] + 63) / 64;
 
     int st = EPI_OK;
-    DevBuf dgeno, dorder, dbed;
+    DevBuf &dgeno = ctx->cgeno, &dorder = ctx->corder, &dbed = ctx->cbed;
     const size_t gbytes = (size_t)n_samples * n_snps;
     const size_t bbytes = bed ? (size_t)((n_samples + 3) / 4) * n_snps : 0;
     st = dgeno.reserve(gbytes);
@@ -303,31 +303,31 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
         st = co->desc.reserve(sizeof(CohortDev));
     hipError_t e = hipSuccess;
     if (st == EPI_OK && bed) {
-        e = hipMemcpyAsync(dbed.ptr, bed, bbytes, hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(dbed.ptr, bed, bbytes, hipMemcpyHostToDevice, strm);
         if (e == hipSuccess)
             e = launch_bed_unpack((const uint8_t *)dbed.ptr, n_samples, n_snps,
-                                  (uint8_t *)dgeno.ptr, ctx->stream);
+                                  (uint8_t *)dgeno.ptr, strm);
     } else if (st == EPI_OK) {
-        e = hipMemcpyAsync(dgeno.ptr, geno, gbytes, hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(dgeno.ptr, geno, gbytes, hipMemcpyHostToDevice, strm);
     }
     if (st == EPI_OK && e == hipSuccess)
         e = hipMemcpyAsync(dorder.ptr, order.data(), (size_t)kept * 4, hipMemcpyHostToDevice,
-                           ctx->stream);
+                           strm);
     if (st == EPI_OK && e == hipSuccess)
         e = launch_pack((const uint8_t *)dgeno.ptr, n_samples, n_snps, co->Mp,
                         (const uint32_t *)dorder.ptr, co->n[0], co->n[1],
                         (uint4 *)co->sm[0].ptr, (uint4 *)co->wm[0].ptr,
                         (uint4 *)co->sm[1].ptr, (uint4 *)co->wm[1].ptr, co->W[0], co->W[1],
-                        ctx->stream);
+                        strm);
     CohortDev h = co->host_desc();
     if (st == EPI_OK && e == hipSuccess)
-        e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream);
+        e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm);
     // the pair and single margins are part of the resident cohort: every
     // scan reads them, none recomputes them
     if (st == EPI_OK && e == hipSuccess)
-        e = launch_pairs((const CohortDev *)co->desc.ptr, h, ctx->stream);
+        e = launch_pairs((const CohortDev *)co->desc.ptr, h, strm);
     if (st == EPI_OK && e == hipSuccess)
-        e = hipStreamSynchronize(ctx->stream);
+        e = hipStreamSynchronize(strm);
     if (st == EPI_OK && e != hipSuccess)
         st = hip_status(e);
     if (st != EPI_OK) {
@@ -351,7 +351,8 @@ extern "C" int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_
         return EPI_EINVAL;
     *out = nullptr;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    return epi::cohort_create_locked(ctx, geno, nullptr, pheno, n_samples, n_snps, out);
+    return epi::cohort_create_locked(ctx, geno, nullptr, pheno, n_samples, n_snps, out,
+                                     ctx->stream);
 }
 
 extern "C" int epi_cohort_create_bed(epi_ctx *ctx, const uint8_t *bed, const uint8_t *pheno,
@@ -362,7 +363,8 @@ extern "C" int epi_cohort_create_bed(epi_ctx *ctx, const uint8_t *bed, const uin
     if (!ctx || !bed || !pheno || !out || !epi::cohort_shape_ok(n_samples, n_snps))
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    return epi::cohort_create_locked(ctx, nullptr, bed, pheno, n_samples, n_snps, out);
+    return epi::cohort_create_locked(ctx, nullptr, bed, pheno, n_samples, n_snps, out,
+                                     ctx->stream);
 }
 
 CohortDev epi_cohort::host_desc() const

@@ -88,9 +88,11 @@ void tune_load_default(epi_ctx *ctx);
 bool tune_lookup(const epi_ctx *ctx, uint32_t n_samples, ScanConfig &cfg);
 bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps);
 uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps);
+// Packs a cohort into device memory on stream `s` (synchronized before
+// returning).  Caller holds ctx->mu.
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out);
+                         epi_cohort **out, hipStream_t s);
 void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<double> &t64);
 int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);
@@ -152,6 +154,10 @@ struct epi_ctx {
     // per-launch scratch, grown on demand
     epi::DevBuf round, fin, counters, slots, rescored;
     epi::DevBuf lists;  // running top-kTop lists of the jobs of one call
+    // cohort creation inputs on the device (genotypes, sample order, .bed
+    // bytes), kept so that creating a cohort frees nothing (a free waits for
+    // the device, which may be running a scan of other cohorts)
+    epi::DevBuf cgeno, corder, cbed;
     epi::HostBuf stage;
     hipEvent_t stage_ev = nullptr;  // marks the last copy out of `stage`
     bool stage_pending = false;

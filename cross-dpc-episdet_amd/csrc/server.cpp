@@ -3,16 +3,18 @@
 // No reference counterpart: the reference scans one generated dataset per
 // process (GPU/v4.cpp:198-624).  Here a worker thread owns the context's
 // stream and runs scheduling rounds:
-//   1. admission: queued requests are packed into HBM (cohort planes, pair
-//      margins, resident row table) in FIFO order while the device
-//      footprint stays within the budget and fewer than max_active run;
-//   2. one round: every active request gets a fair share of a work budget
+//   1. one round: every active request gets a fair share of a work budget
 //      (sized from the measured throughput to take about round_ms of device
 //      time) as a contiguous range of its remaining tile rows; the ranges
 //      of all requests become the segments of one persistent scan launch,
 //      replayed from a captured HIP graph (queue reset, scan, merge);
-//   3. requests whose rows are exhausted are rescored and completed; their
-//      device memory is released before the next admission.
+//   2. admission, while that round runs on the device: queued requests are
+//      packed into HBM (cohort planes and pair margins on a stream of the
+//      server's own, then the resident row table) in FIFO order while the
+//      device footprint stays within the budget and fewer than max_active
+//      run; they join the next round;
+//   3. after the round, requests whose rows are exhausted are rescored and
+//      completed, and their device memory is released.
 // Every triplet of a request is scored exactly once whatever the round
 // split, and each round folds the new candidates into the request's exact
 // running top-64, so a result is identical to epi_scan of the same data.
@@ -86,9 +88,11 @@ struct epi_server {
     double units_per_ms = 1e5;  // throughput estimate, refined by every round
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
+    hipStream_t upload = nullptr;  // cohort creation, overlapping the rounds
 
     void run();
-    int round(std::vector<Request *> &act);
+    int round_launch(std::vector<Request *> &act, double &units);
+    int round_wait(double units);
     int admit_locked_ctx(Request *r);
     void release_locked_ctx(Request *r);
     void complete(Request *r, int status);
@@ -120,7 +124,7 @@ void epi_server::complete(Request *r, int status)
 int epi_server::admit_locked_ctx(Request *r)
 {
     int st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
-                                  &r->co);
+                                  &r->co, upload);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg, nullptr);
     r->admitted = Clock::now();
@@ -141,8 +145,9 @@ void epi_server::release_locked_ctx(Request *r)
     r->job.rows.shrink_to_fit();
 }
 
-// One scheduling round over `act` (ctx->mu held).  Returns the device time.
-int epi_server::round(std::vector<Request *> &act)
+// Launches one scheduling round over `act` (ctx->mu held); `units` is its
+// work estimate.
+int epi_server::round_launch(std::vector<Request *> &act, double &units)
 {
     const uint32_t cap = cfg.max_active;
     const uint32_t n = (uint32_t)act.size();
@@ -151,7 +156,7 @@ int epi_server::round(std::vector<Request *> &act)
     const double share = budget / std::max<uint32_t>(n, 1);
     std::vector<Job *> jp(n);
     std::vector<std::pair<size_t, size_t>> ranges(n);
-    double units = 0.0;
+    units = 0.0;
     for (uint32_t b = 0; b < n; ++b) {
         Job &J = act[b]->job;
         jp[b] = &J;
@@ -198,6 +203,13 @@ int epi_server::round(std::vector<Request *> &act)
     EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     EPI_HIP(hipGraphLaunch(graph, ctx->stream));
     EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    return EPI_OK;
+}
+
+// Waits for the launched round (ctx->mu held) and refines the throughput
+// estimate from its device time.
+int epi_server::round_wait(double units)
+{
     EPI_HIP(hipEventSynchronize(ctx->ev1));
     ctx->stage_pending = false;
     float dms = 0.0f;
@@ -247,28 +259,35 @@ void epi_server::run()
             std::lock_guard<std::mutex> g(ctx->mu);
             if (hipSetDevice(ctx->device) != hipSuccess)
                 break;
+            // the round of the requests already active
+            std::vector<Request *> act;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                for (Request *r : active)
+                    if (!(r->cancel || stop))
+                        act.push_back(r);
+            }
+            double units = 0.0;
+            int st = act.empty() ? EPI_OK : round_launch(act, units);
+            const bool launched = !act.empty() && st == EPI_OK;
+            // admission while it runs (the new requests join the next round)
             for (Request *r : admit) {
-                const int st = admit_locked_ctx(r);
-                if (st != EPI_OK) {
+                const int as = admit_locked_ctx(r);
+                if (as != EPI_OK) {
+                    (void)hipStreamSynchronize(upload);
                     (void)hipStreamSynchronize(ctx->stream);
                     ctx->stage_pending = false;
                     release_locked_ctx(r);
-                    r->status = st;
+                    r->status = as;
                     failed.push_back(r);
                     continue;
                 }
                 std::lock_guard<std::mutex> lk(m);
                 active.push_back(r);
-            }
-            std::vector<Request *> act;
-            {
-                std::lock_guard<std::mutex> lk(m);
                 stats.active = (uint32_t)active.size();
-                for (Request *r : active)
-                    if (!(r->cancel || stop))
-                        act.push_back(r);
             }
-            int st = act.empty() ? EPI_OK : round(act);
+            if (launched)
+                st = round_wait(units);
             // requests that are cancelled, finished, or hit by a failed round
             std::vector<Request *> keep, done_ok, drop;
             {
@@ -352,6 +371,7 @@ extern "C" int epi_server_create(epi_ctx *ctx, const epi_server_config *config,
         EPI_TRY(ctx_ensure_tables(ctx, cfg.max_samples));
         s->scfg = resolve_config(ctx, nullptr, cfg.max_samples);
         EPI_TRY(plan_launch(ctx, s->scfg, cfg.max_samples + 2, false, s->plan));
+        EPI_HIP(hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking));
     }
     s->ctx = ctx;
     s->cfg = cfg;
@@ -377,9 +397,12 @@ extern "C" int epi_server_destroy(epi_server *s)
     s->cv_work.notify_all();
     if (s->worker.joinable())
         s->worker.join();
-    if (s->graph) {
+    {
         std::lock_guard<std::mutex> g(s->ctx->mu);
-        (void)hipGraphExecDestroy(s->graph);
+        if (s->graph)
+            (void)hipGraphExecDestroy(s->graph);
+        if (s->upload)
+            (void)hipStreamDestroy(s->upload);
     }
     delete s;
     return EPI_OK;
