@@ -86,3 +86,14 @@ def test_mfma_batch_equals_separate(epi, ctx, var):
     finally:
         for co in cos:
             co.close()
+
+
+def test_kernels_alternate_on_one_cohort(epi, ctx):
+    """One cohort scanned by the three kernels in turn: its cached tile rows
+    are rebuilt whenever the tile width changes, and every scan is exact."""
+    g, p = epi.generate_hwe(55, 3000, 200, 1400)
+    with ctx.cohort(g, p) as co:
+        runs = [ctx.scan(co, tile={"variant": v}) for v in (3, 1, 2, 3, 1)]
+    ref = [(c.i, c.j, c.k, c.score) for c in runs[0].top]
+    for r in runs[1:]:
+        assert [(c.i, c.j, c.k, c.score) for c in r.top] == ref

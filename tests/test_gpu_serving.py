@@ -49,6 +49,21 @@ def test_batch_graph_replay(epi, ctx):
             c.close()
 
 
+def test_batch_repeated_cohort(epi, ctx):
+    """A cohort may appear more than once in a batch: its cached row table is
+    shared, each entry gets its own running list, every result is exact."""
+    data = make_cohorts(epi, ctx, 3, 21)
+    cos = [ctx.cohort(g, p) for g, p in data]
+    try:
+        single = [ctx.scan(c) for c in cos]
+        order = [0, 1, 0, 2, 1, 0]
+        batch = ctx.scan_batch([cos[b] for b in order])
+        assert all(same(single[b], r) for b, r in zip(order, batch))
+    finally:
+        for c in cos:
+            c.close()
+
+
 def test_server_results_equal_plain_scans(epi, ctx):
     data = make_cohorts(epi, ctx, 16, 9, (10, 400), (64, 6000))
     plain = []
