@@ -10,11 +10,12 @@
 // in one of two number formats, both exact:
 //   int8 (variant 2): 0/1 bytes, v_mfma_i32_32x32x32_i8, int32 accumulation;
 //   FP4  (variant 3): e2m1 nibbles 0 and 1.0 (0x2), two samples per byte,
-//        v_mfma_scale_f32_32x32x64_f8f6f4 (both block scales 2^0), FP32
-//        accumulation.  Every product is 0 or 1 and every partial sum an
+//        v_mfma_scale_f32_32x32x64_f8f6f4 (block scale 2^0 on A, 2^2 on B),
+//        FP32 accumulation.  Every product is 0 or 4 and every partial sum an
 //        integer below 2^24 (class sizes are below 2^21), so the FP32 sums are
-//        exact; the instruction does twice the int8 instruction's MACs in the
-//        same issue time, and the operands take half the bytes.
+//        exact (4 x the counts); the instruction does twice the int8
+//        instruction's MACs in the same issue time, and the operands take
+//        half the bytes.
 // The operand bytes come from a per-cohort expansion of the bit planes
 // (E[c] or F[c], [snp][plane][sample]).  Both formats use one LDS layout:
 // 128-byte rows per stage, 32 bytes per MFMA step (32 samples as bytes, 64
@@ -22,18 +23,17 @@
 // over the K dimension covers each sample once whatever that mapping is.
 //
 // Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
-// 4 wavefronts.  Per stage of 128 samples the workgroup stages the j-block
-// and k-block bytes (and SNP i's) in LDS; wavefront w computes the 64 rows
+// 4 wavefronts, one per SIMD.  Per stage the workgroup copies the j-block
+// and k-block rows (and SNP i's) into LDS; wavefront w computes the 64 rows
 // (16 j x 4 (a,b)) of its quarter against all 128 columns (64 k x 2 c):
-// 2 x 4 MFMA tiles of 32 x 32 per class, 256 int32 accumulators.
-// Pipelining: two LDS stage buffers, so one barrier per stage; the next
-// stage is copied from global memory straight into the other buffer
-// (global_load_lds, no registers) while this stage's MFMAs run; within a
-// stage the operands of the next 32-sample step are read from LDS while the
-// current step's MFMAs run, and a stage's last step runs after the barrier
-// together with the next stage's first reads.  The epilogue is the VALU kernel's: exact 27-cell
-// reconstruction from the margins, K2 in the reference order, per-wavefront
-// top-64.
+// 2 x 4 MFMA tiles of 32 x 32 per class, 256 accumulators in AGPRs.
+// Pipelining: two LDS stage buffers, one barrier per stage.  Steps 0 and 1
+// of a stage run before its barrier, steps 2 and 3 after it (operands in
+// registers), and the next stage's copies (global_load_lds, no registers)
+// are issued between those MFMAs; each step's operand reads are issued a
+// step ahead.  The epilogue rebuilds the 27-cell tables exactly from the 8
+// counted cells and the pair and single margins, sums K2 in the reference
+// order and keeps a per-wavefront top-64.
 #include <math.h>
 
 #include "episdet_cells.h"
