@@ -157,6 +157,32 @@ def batched_leg(epi, ctx, seed=4, n=64):
             "speedup_vs_sequential": t_plain / t_batch, "identical_to_plain": ok}
 
 
+def configs_leg(epi, ctx):
+    """The two smaller configurations: 256 SNPs x 1024 samples (HWE) and
+    2048 SNPs x 4096 samples with one planted 3-way interaction; FP32 vs
+    FP64 score error of the top 64 and recovery of the planted triplet."""
+    out = []
+    for name, M, N, planted in (("256x1024 HWE", 256, 1024, False),
+                                ("2048x4096 planted", 2048, 4096, True)):
+        if planted:
+            g, p, pl = epi.generate_planted(11, N, M, N // 2)
+        else:
+            g, p = epi.generate_hwe(1, N, M, N // 2)
+            pl = None
+        with ctx.cohort(g, p) as co:
+            ctx.scan(co)  # warm-up (row table, expanded planes)
+            r = ctx.scan(co)
+        out.append({"config": name, "snps": M, "samples": N, "scan_ms": r.scan_ms,
+                    "triplets_x_samples_per_s": trip(M) * N / (r.scan_ms / 1e3),
+                    "best": [r.i, r.j, r.k], "score": r.score, "score_fp64": r.score_fp64,
+                    "max_abs_err_fp32_top64": r.max_abs_err_fp32,
+                    "fp64_order_differs": bool(r.fp64_order_differs),
+                    "planted": list(pl) if pl else None,
+                    "planted_recovered": (list(pl) == [r.i, r.j, r.k]) if pl else None})
+        log(f"{name}: best {out[-1]['best']} err {r.max_abs_err_fp32:.2e}")
+    return out
+
+
 def serving_leg(epi, ctx, capacity, seed=5, n=64, load=0.8):
     """Poisson stream through the server vs FIFO plain scans."""
     import numpy as np
@@ -287,6 +313,7 @@ def main():
         device = ctx.device_name
         capacity = M * (M - 1) * (M - 2) // 6 * N * args.steps / elapsed
         batched = serving = None
+        others = configs_leg(epi, ctx)
         if not args.no_batch:
             log("batched leg: 64 uneven cohorts")
             batched = batched_leg(epi, ctx)
@@ -311,6 +338,12 @@ def main():
             for e in json.load(f).get("kernels", []):
                 if e.get("variant") == variant:
                     traffic = e.get("hbm_bytes_per_launch")
+    single = ntrip * N / kernel_s  # the scan kernel's own rate
+    popcount_peak = VALU_PEAK_OPS * 32 / 54  # triplets x samples/s at 54 lane-ops per 32 samples
+    if batched:
+        batched["frac_of_single_scan"] = batched["value"] / value
+    if serving:
+        serving["frac_of_single_scan"] = serving["value"] / value
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -341,7 +374,12 @@ def main():
                      "executed": {"achieved": executed, "frac": executed / peak,
                                   "ops_per_unit": work["executed_ops"],
                                   "note": "ops the kernel issues: 8 of 27 cells counted, "
-                                          "19 derived from margins"}},
+                                          "19 derived from margins"},
+                     "traffic_gbs": traffic / kernel_s / 1e9 if traffic else None},
+        # the reference's own bound: integer VALU AND + popcount, 27 cells x
+        # (v_bitop3 + v_bcnt) per 32 samples on 256 CU x 64 lanes x 2.4 GHz
+        "popcount_roofline": {"peak": popcount_peak, "unit": "triplets*samples/s",
+                              "achieved": single, "frac": single / popcount_peak},
         # request latency of the serving stream when it ran, else of the scans above
         "latency_ms": (serving["latency_ms"] if serving else
                        {"p50": pct(lat, 0.5), "p99": pct(lat, 0.99)}),
@@ -353,6 +391,7 @@ def main():
         "variants": compare,
         "batched": batched,
         "serving": serving,
+        "other_configs": others,
     }
     print(json.dumps(out), flush=True)
 
