@@ -87,12 +87,46 @@ void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<doubl
     }
 }
 
+void *DevPool::take(size_t want, size_t &got)
+{
+    auto it = blocks.lower_bound(want);
+    if (it == blocks.end() || it->first > want + want / 4 + ((size_t)1 << 20))
+        return nullptr;
+    got = it->first;
+    void *p = it->second;
+    held -= got;
+    blocks.erase(it);
+    return p;
+}
+
+void DevPool::give(void *p, size_t bytes)
+{
+    if (held + bytes > limit) {
+        (void)hipFree(p);
+        return;
+    }
+    blocks.emplace(bytes, p);
+    held += bytes;
+}
+
+void DevPool::trim()
+{
+    for (auto &b : blocks)
+        (void)hipFree(b.second);
+    blocks.clear();
+    held = 0;
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release()
 {
-    if (ptr)
-        (void)hipFree(ptr);
+    if (ptr) {
+        if (pool)
+            pool->give(ptr, bytes);
+        else
+            (void)hipFree(ptr);
+    }
     ptr = nullptr;
     bytes = 0;
 }
@@ -104,7 +138,21 @@ int DevBuf::reserve(size_t want)
     release();
     if (want == 0)
         return EPI_OK;
+    if (pool) {
+        size_t got = 0;
+        if (void *p = pool->take(want, got)) {
+            ptr = p;
+            bytes = got;
+            return EPI_OK;
+        }
+    }
     hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess && pool && pool->held) {
+        // the kept blocks may be what is missing
+        (void)hipGetLastError();
+        pool->trim();
+        e = hipMalloc(&ptr, want);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         return hip_status(e);
@@ -156,6 +204,8 @@ extern "C" int epi_open(int device, epi_ctx **out)
         return EPI_EDEVICE;
     }
     ctx->cu_count = prop.multiProcessorCount;
+    // destroyed cohorts' blocks are kept up to a quarter of the device
+    ctx->pool.limit = prop.totalGlobalMem / 4;
     ctx->lds_per_block = prop.sharedMemPerBlock;
     char name[256] = {0};
     if (hipDeviceGetName(name, (int)sizeof(name) - 1, device) == hipSuccess && name[0])
@@ -224,6 +274,15 @@ epi_cohort::~epi_cohort()
     desc.release();
 }
 
+void epi_cohort::use_pool(epi::DevPool *p)
+{
+    for (int c = 0; c < 2; ++c) {
+        sm[c].pool = wm[c].pool = pair[c].pool = single[c].pool = p;
+        E[c].pool = F[c].pool = p;
+    }
+    desc.pool = drows.pool = p;
+}
+
 namespace epi {
 
 bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps)
@@ -272,6 +331,7 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     if (!co)
         return EPI_ENOMEM;
     co->ctx = ctx;
+    co->use_pool(&ctx->pool);
     co->N = kept;
     co->M = n_snps;
     co->Mp = (n_snps + 63) / 64 * 64;
@@ -331,6 +391,7 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     if (st == EPI_OK && e != hipSuccess)
         st = hip_status(e);
     if (st != EPI_OK) {
+        (void)hipStreamSynchronize(strm);  // nothing in flight may use the blocks
         delete co;
         return st;
     }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -26,10 +27,32 @@
 
 namespace epi {
 
+// Device blocks that destroyed cohorts gave back, kept for the cohorts
+// created after them.  A hipMalloc / hipFree pair of a large cohort buffer
+// costs up to milliseconds of host time, and a free waits for the whole
+// device, which a server running scans of other cohorts cannot afford.  A
+// block is given back only when no queued work reads it (the callers have
+// synchronized the stream), and every call is made with ctx->mu held.
+struct DevPool {
+    std::multimap<size_t, void *> blocks;  // by size
+    size_t held = 0;
+    size_t limit = 0;  // most bytes kept; a block beyond it is freed
+    DevPool() = default;
+    DevPool(const DevPool &) = delete;
+    DevPool &operator=(const DevPool &) = delete;
+    ~DevPool() { trim(); }
+    // A kept block of at least `want` bytes and at most about a quarter
+    // more (its size in `got`), or null.
+    void *take(size_t want, size_t &got);
+    void give(void *p, size_t bytes);
+    void trim();  // frees every kept block
+};
+
 // Owned device allocation that only grows.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    DevPool *pool = nullptr;  // where release() gives the block (null: hipFree)
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -141,6 +164,7 @@ int finish_jobs(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
 
 struct epi_ctx {
     int device = 0;
+    epi::DevPool pool;  // blocks of destroyed cohorts
     int cu_count = 0;
     size_t lds_per_block = 65536;
     std::string name, arch;
@@ -184,5 +208,6 @@ struct epi_cohort {
     epi::DevBuf drows;
     uint64_t bytes = 0;
     epi::CohortDev host_desc() const;
+    void use_pool(epi::DevPool *p);
     ~epi_cohort();
 };

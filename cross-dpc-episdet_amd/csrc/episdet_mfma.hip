@@ -657,6 +657,18 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll 1
         for (int t = 0; t < 4; ++t) {
             const int fa = t >> 1, kq = t & 1;
+            {
+                // The wavefront's j of this (fa, kq) are [jlo, jlo + 8), its
+                // k [klo, klo + 32): skip it when no i < j < k < M is among
+                // them (SNP i inside the j-block, the lower half of a block
+                // on the diagonal, k-blocks past M).
+                const uint32_t jlo = J0 + 16 * wave + 8 * (uint32_t)fa;
+                const uint32_t klo = K0 + 32 * (uint32_t)kq;
+                const uint32_t jmin = jlo > i + 1 ? jlo : i + 1;
+                const uint32_t khi = klo + 31 < M - 1 ? klo + 31 : M - 1;
+                if (klo >= M || jmin > jlo + 7 || jmin >= khi)
+                    continue;
+            }
             uint32_t v[2][2][16];  // [class][cc][e], 4 x counts
             switch (t) {
 #define EPI_MF_COPY(T, FA, KQ)                                                       \
@@ -708,12 +720,14 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 const uint32_t jl = 16 * wave + 8 * (uint32_t)fa + 2 * sl + half;
                 const uint32_t j = J0 + jl;
                 const bool valid = (i < j) && (j < k) && (k < M);
-                // Only lanes with i < j < k are scored (a slot with none is
-                // skipped).  The pair table holds every (p, q), so counts are
-                // exact for every lane and every offset below is at most
-                // 4 (N + 1), inside the table.
+                // Only lanes with i < j < k are offered; a slot with none
+                // is skipped by a wave-uniform branch (a per-lane `if` is
+                // turned into a select by the compiler, which then scores
+                // every slot).  The pair table holds every (p, q), so counts
+                // are exact for every lane and every offset below is at
+                // most 4 (N + 1), inside the table.
                 float sc = 0.0f;
-                if (valid) {
+                if (__ballot(valid) != 0) {
                     uint32_t x[2][27];
 #pragma unroll
                     for (int c = 0; c < 2; ++c) {
@@ -734,7 +748,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                         const uint32_t ask[2] = {sk[c].x, sk[c].y};
                         derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
                     }
-                    if (DUMP && P.dump_counts) {
+                    if (DUMP && P.dump_counts && valid) {
                         const uint64_t rk = dump_rank(M, i, j, k);
 #pragma unroll
                         for (int m = 0; m < 27; ++m) {

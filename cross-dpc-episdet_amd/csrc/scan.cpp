@@ -196,6 +196,7 @@ int job_init(epi_ctx *ctx, Job &job, epi_cohort *co, const ScanConfig &cfg, Cand
     if (cfg.variant >= 2)
         EPI_TRY(ensure_expanded(ctx, co, cfg.variant));
     if (!list) {
+        job.own_list.pool = &ctx->pool;
         EPI_TRY(job.own_list.reserve(sizeof(Cand) * kTop));
         list = (Cand *)job.own_list.ptr;
         EPI_HIP(launch_fill_empty(list, kTop, ctx->stream));
