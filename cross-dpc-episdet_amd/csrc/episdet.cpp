@@ -268,9 +268,9 @@ epi_cohort::~epi_cohort()
         wm[c].release();
         pair[c].release();
         single[c].release();
-        E[c].release();
-        F[c].release();
     }
+    E.release();
+    F.release();
     desc.release();
 }
 
@@ -278,9 +278,8 @@ void epi_cohort::use_pool(epi::DevPool *p)
 {
     for (int c = 0; c < 2; ++c) {
         sm[c].pool = wm[c].pool = pair[c].pool = single[c].pool = p;
-        E[c].pool = F[c].pool = p;
     }
-    desc.pool = drows.pool = p;
+    E.pool = F.pool = desc.pool = drows.pool = p;
 }
 
 namespace epi {
@@ -290,15 +289,20 @@ bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps)
     return n_samples > 0 && n_snps >= 3 && n_snps <= (1u << 21) - 1;
 }
 
-// Device bytes a cohort of this shape may hold (planes in two layouts, pair
-// and single margins, descriptor, and the 0/1-byte expansion the MFMA scan
-// adds: 2 planes x Mp rows x (n_c rounded up to 128) bytes per class), the
+// Device bytes a cohort of this shape may hold when scanned by `variant`
+// (planes in two layouts, pair and single margins, descriptor, and the
+// expansion an MFMA scan adds: 2 classes x 2 planes x Mp rows of one pitch,
+// the larger class's samples rounded up to a 128-byte stage, at most
+// n_samples / 2 + 128 bytes as nibbles and n_samples + 128 as bytes), the
 // unit of server admission.
-uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps)
+uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant)
 {
     const uint64_t Mp = (n_snps + 63) / 64 * 64;
     const uint64_t words = (n_samples + 63) / 64 + 2;  // both classes, rounded up
-    const uint64_t expanded = 2 * Mp * ((uint64_t)n_samples + 256);
+    const uint64_t N = n_samples;
+    const uint64_t expanded = variant == 3 ? 4 * Mp * (N / 2 + 128)
+                            : variant == 2 ? 4 * Mp * (N + 128)
+                                           : 0;
     return 2 * words * Mp * 16 + 2 * Mp * Mp * 16 + 2 * Mp * 8 + expanded + sizeof(CohortDev) +
            4096;
 }
@@ -437,15 +441,19 @@ CohortDev epi_cohort::host_desc() const
         h.wm[c] = (const uint4 *)wm[c].ptr;
         h.pair[c] = (uint4 *)pair[c].ptr;
         h.single[c] = (uint2 *)single[c].ptr;
-        h.E[c] = (const uint8_t *)E[c].ptr;
-        h.EL[c] = EL[c];
-        h.F[c] = (const uint8_t *)F[c].ptr;
-        h.FL[c] = FL[c];
+        h.E[c] = E.ptr ? (const uint8_t *)E.ptr + c * (size_t)Mp * 2 * EL : nullptr;
+        h.EL[c] = EL;
+        h.F[c] = F.ptr ? (const uint8_t *)F.ptr + c * (size_t)Mp * 2 * FL : nullptr;
+        h.FL[c] = FL;
         h.W[c] = W[c];
         h.n[c] = n[c];
     }
     h.M = M;
     h.Mp = Mp;
+    // class 1's planes as a byte offset from class 0's, when it fits 32 bits
+    const size_t e1 = (size_t)Mp * 2 * EL, f1 = (size_t)Mp * 2 * FL;
+    h.E1off = e1 < ((size_t)1 << 32) ? (uint32_t)e1 : 0;
+    h.F1off = f1 < ((size_t)1 << 32) ? (uint32_t)f1 : 0;
     return h;
 }
 

@@ -37,9 +37,10 @@ struct CohortDev {
     uint4 *pair[2];       // [Mp][Mp], every (p, q): {n00, n01, n10, n11}
     uint2 *single[2];     // [Mp]: {count g0, count g1}
     const uint8_t *E[2];  // int8 MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
-    uint32_t EL[2];       // bytes per E row: 64 W[c] rounded up to 128, zero padded
+    uint32_t EL[2];       // bytes per E row, both classes: 64 max(W) rounded up to 128, zero padded
     const uint8_t *F[2];  // FP4 MFMA variant: e2m1 nibbles 0 / 1.0 [Mp][2 planes][FL[c]], or null
-    uint32_t FL[2];       // bytes per F row: 32 W[c] rounded up to 128, zero padded
+    uint32_t FL[2];       // bytes per F row, both classes: 32 max(W) rounded up to 128, zero padded
+    uint32_t E1off, F1off;  // E[1] - E[0], F[1] - F[0] in bytes (0: too far for 32 bits)
     uint32_t W[2];        // 64-bit words per class
     uint32_t n[2];        // class sizes
     uint32_t M, Mp;       // SNPs, padded to a multiple of 64

@@ -110,7 +110,7 @@ int tune_load_file(epi_ctx *ctx, const std::string &path);
 void tune_load_default(epi_ctx *ctx);
 bool tune_lookup(const epi_ctx *ctx, uint32_t n_samples, ScanConfig &cfg);
 bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps);
-uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps);
+uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant);
 // Packs a cohort into device memory on stream `s` (synchronized before
 // returning).  Caller holds ctx->mu.
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
@@ -197,10 +197,11 @@ struct epi_cohort {
     uint32_t n[2] = {0, 0};
     uint32_t W[2] = {0, 0};
     epi::DevBuf sm[2], wm[2], pair[2], single[2], desc;
-    epi::DevBuf E[2];  // 0/1-byte planes for the int8 MFMA variant, built on first use
-    uint32_t EL[2] = {0, 0};  // their row pitch in bytes
-    epi::DevBuf F[2];  // e2m1 nibble planes for the FP4 MFMA variant, built on first use
-    uint32_t FL[2] = {0, 0};  // their row pitch in bytes
+    // expanded planes of both classes, class 1 right after class 0, built on
+    // first use: 0/1 bytes for the int8 MFMA variant (E), e2m1 nibbles for
+    // the FP4 one (F); rows of one pitch (EL, FL bytes) in both classes
+    epi::DevBuf E, F;
+    uint32_t EL = 0, FL = 0;
     // triplet-tile rows for tile width rows_jp (0: none yet), built on first use
     std::vector<epi::Row> rows;
     uint64_t rows_items = 0;

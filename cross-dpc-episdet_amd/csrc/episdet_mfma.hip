@@ -71,10 +71,11 @@ __device__ __forceinline__ uint2 ldc2(const uint2 *p)
 }  // namespace mf
 
 // ------------------------------------------------------------------------
-// bit planes -> 0/1 bytes: E[c] row (snp, plane) holds EL bytes (64 W_c
-// rounded up to a whole 128-byte stage), byte t = bit t of the plane
-// (sample t of the class); padding bytes are 0.  One thread per (snp,
-// 64-bit word slot, plane) writes 64 bytes; slots w >= W write zeros.
+// bit planes -> 0/1 bytes: E[c] row (snp, plane) holds EL bytes (64 W of
+// the larger class rounded up to a whole 128-byte stage: both classes have
+// one row pitch), byte t = bit t of the plane (sample t of the class);
+// padding bytes are 0.  One thread per (snp, 64-bit word slot, plane) writes
+// 64 bytes; slots w >= W write zeros.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 expand_kernel(const uint4 *__restrict__ sm, uint32_t Mp, uint32_t W, uint32_t EL,
@@ -117,9 +118,10 @@ hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, 
     return hipGetLastError();
 }
 
-// bit planes -> e2m1 nibbles: F[c] row (snp, plane) holds FL bytes (32 W_c
-// rounded up to a whole 128-byte stage); sample t is nibble t % 8 of dword
-// t / 8, 0x2 (1.0) when bit t of the plane is set, else 0; padding is 0.
+// bit planes -> e2m1 nibbles: F[c] row (snp, plane) holds FL bytes (32 W of
+// the larger class rounded up to a whole 128-byte stage, as for E); sample
+// t is nibble t % 8 of dword t / 8, 0x2 (1.0) when bit t of the plane is
+// set, else 0; padding is 0.
 // One thread per (snp, 64-bit word slot, plane) writes 32 bytes.
 __global__ void __launch_bounds__(256)
 expand_fp4_kernel(const uint4 *__restrict__ sm, uint32_t Mp, uint32_t W, uint32_t FL,
@@ -531,20 +533,37 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                         acc[c][fa][fb][e] = 0;
 
         // Stage loop (see the pipelining note at the top of this file).
+        // Both classes' rows have one pitch (bytes per (snp, plane) row, a
+        // multiple of kStage, zero padded) and class 1's planes follow class
+        // 0's in one allocation; class c has its own stage count (256
+        // samples per stage as nibbles, 128 as bytes).  The last stage of
+        // class 0 copies class 1's first stage (a byte offset jb further),
+        // so class 1 starts without waiting for memory.
+        uint32_t nst[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            nst[c] = FP4 ? (C->W[c] + 3) / 4 : (C->W[c] + 1) / 2;
+        uint32_t par = 0;     // stage buffer of the class's first stage
+        bool staged = false;  // class 1's first stage is in stage_lds[par]
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const uint8_t *E = FP4 ? C->F[c] : C->E[c];
-            const uint32_t Lc = FP4 ? C->FL[c] : C->EL[c];  // bytes per (snp, plane) row, a multiple of kStage
-            if (Lc == 0)
+            const uint32_t Lc = FP4 ? C->FL[c] : C->EL[c];
+            const uint32_t n_st = nst[c];
+            if (n_st == 0)
                 continue;
-            const uint32_t n_st = Lc / kStage;
-            // Both buffers are free here: the previous stage loop ended with
-            // a barrier, and the previous item's epilogue (which reuses them
-            // for margins) ended with the barrier before the next claim.
+            const uint32_t jb = c == 0 && nst[1] > 0 ? (FP4 ? C->F1off : C->E1off) : 0u;
             const MfCopyOffsets co = mf_copy_offsets(Lc, wave, lane);
             const uint8_t *blk_base = E + (size_t)((wave >> 1) ? K0 : J0) * 2 * Lc;
             const uint8_t *i_base = E + (size_t)i * 2 * Lc;
-            mf_issue_stage(stage_lds[0], blk_base, i_base, 0u, co, wave, lane);
+            if (!(c == 1 && staged)) {
+                // Both buffers are free here: the previous stage loop ended
+                // with a barrier, and the previous item's epilogue (which
+                // reuses them for margins) ended with the barrier before the
+                // next claim.
+                par = 0;
+                mf_issue_stage(stage_lds[0], blk_base, i_base, 0u, co, wave, lane);
+            }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             // Steps 2 and 3 of a stage run after the barrier that ends it
@@ -556,13 +575,14 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             // the reads of the following step's operands issued ahead.
             v4i dA[2][2], dB[2][4];  // the deferred steps' A (masked by SNP i) and B
             for (uint32_t st = 0; st < n_st; ++st) {
-                const uint32_t sb = stage_lds[st & 1];
+                const uint32_t sb = stage_lds[(st + par) & 1];
                 // The next stage's copies go to the buffer the last stage
-                // read.  In the last stage they copy this stage again (valid
-                // rows, an idle buffer) rather than branch around the copies;
-                // like every copy they land before this stage's barrier.
-                const uint32_t nsb = stage_lds[(st + 1) & 1];
-                const uint32_t ns0 = (st + 1 < n_st ? st + 1 : st) * kStage;
+                // read.  In the last stage they copy class 1's first stage
+                // (class 0, jb != 0) or this stage again (valid rows, an idle
+                // buffer) rather than branch around the copies; like every
+                // copy they land before this stage's barrier.
+                const uint32_t nsb = stage_lds[(st + 1 + par) & 1];
+                const uint32_t ns0 = st + 1 < n_st ? (st + 1) * kStage : (jb ? jb : st * kStage);
                 v4i ei[kStage / 32];
 #pragma unroll
                 for (int kk = 0; kk < kStage / 32; ++kk)
@@ -639,6 +659,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
                         acc[c][fa][fb] = mf_mma(dA[d][fa], dB[d][fb], acc[c][fa][fb]);
+            par = jb ? (n_st & 1) : par;
+            staged = jb != 0;
         }
 
         // ---- epilogue.  Accumulator element e of tile (fa, fb = 2 kq + cc)

@@ -215,25 +215,26 @@ int reserve_lists(epi_ctx *ctx, uint32_t n)
 int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant)
 {
     const bool fp4 = variant == 3;
+    // rows of whole 128-byte stages (128 samples as bytes, 256 as nibbles),
+    // one pitch for both classes so that the kernel's copy offsets serve both
+    const uint32_t Wm = std::max(co->W[0], co->W[1]);
+    const uint32_t L = fp4 ? (Wm + 3) / 4 * 128 : (Wm + 1) / 2 * 128;
+    DevBuf &buf = fp4 ? co->F : co->E;
+    if (buf.ptr)
+        return EPI_OK;
+    const size_t per_class = (size_t)co->Mp * 2 * L;
+    EPI_TRY(buf.reserve(2 * per_class));
+    (fp4 ? co->FL : co->EL) = L;
     for (int c = 0; c < 2; ++c) {
-        DevBuf &buf = fp4 ? co->F[c] : co->E[c];
-        if (buf.ptr)
-            continue;
-        // rows of whole 128-byte stages: 128 samples as bytes, 256 as nibbles
-        const uint32_t L = fp4 ? (co->W[c] + 3) / 4 * 128 : (co->W[c] + 1) / 2 * 128;
-        const size_t bytes = (size_t)co->Mp * 2 * std::max<uint32_t>(L, 128);
-        EPI_TRY(buf.reserve(bytes));
-        (fp4 ? co->FL : co->EL)[c] = L;
-        if (L) {
-            if (fp4)
-                EPI_HIP(launch_expand_fp4((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L,
-                                          (uint8_t *)buf.ptr, ctx->stream));
-            else
-                EPI_HIP(launch_expand((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L,
-                                      (uint8_t *)buf.ptr, ctx->stream));
-        }
-        co->bytes += bytes;
+        uint8_t *dst = (uint8_t *)buf.ptr + c * per_class;
+        if (fp4)
+            EPI_HIP(launch_expand_fp4((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L, dst,
+                                      ctx->stream));
+        else
+            EPI_HIP(launch_expand((const uint4 *)co->sm[c].ptr, co->Mp, co->W[c], L, dst,
+                                  ctx->stream));
     }
+    co->bytes += 2 * per_class;
     return EPI_OK;
 }
 

@@ -414,7 +414,7 @@ extern "C" int epi_submit(epi_server *s, const uint8_t *geno, const uint8_t *phe
     if (!s || !geno || !pheno || !request_id || !cohort_shape_ok(n_samples, n_snps) ||
         n_samples > s->cfg.max_samples)
         return EPI_EINVAL;
-    const uint64_t fp = cohort_footprint(n_samples, n_snps);
+    const uint64_t fp = cohort_footprint(n_samples, n_snps, s->scfg.variant);
     if (fp > s->cfg.memory_budget)
         return EPI_ENOMEM;
     std::unique_ptr<Request> r(new (std::nothrow) Request());
