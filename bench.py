@@ -259,6 +259,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 VALU, 2 int8 MFMA, 3 FP4 MFMA")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the one-scan comparison of the VALU and MFMA kernels")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 256x1024 and planted 2048x4096 configurations")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
@@ -313,7 +315,7 @@ def main():
         device = ctx.device_name
         capacity = M * (M - 1) * (M - 2) // 6 * N * args.steps / elapsed
         batched = serving = None
-        others = configs_leg(epi, ctx)
+        others = None if args.no_configs else configs_leg(epi, ctx)
         if not args.no_batch:
             log("batched leg: 64 uneven cohorts")
             batched = batched_leg(epi, ctx)

@@ -377,12 +377,8 @@ __device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *bl
                    stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
 }
 
-// Stages one j half of the item's (j, k) pair margins, and with half 0 also
-// the (i, j) and (i, k) margins and the single-SNP counts of both blocks,
-// all multiplied by 4 (the epilogue works on 4 x counts, the byte offsets
-// of their ln(n!) entries).
-// Entries with j >= k or i >= j belong to lanes that are scored but never
-// offered; all indices stay below Mp.
+// 4 x each of four counts: the epilogue works on 4 x counts, the byte
+// offsets of their ln(n!) entries.
 __device__ __forceinline__ uint4 x4(uint4 v)
 {
     return make_uint4(v.x << 2, v.y << 2, v.z << 2, v.w << 2);

@@ -15,7 +15,11 @@ def say(*a):
 
 @pytest.mark.parametrize("seed,N,M,cases", [(201, 1024, 256, 512), (202, 777, 70, 300),
                                             (203, 4096, 130, 2048), (204, 65, 67, 30),
-                                            (205, 40000, 24, 20000), (206, 200, 3, 100)])
+                                            (205, 40000, 24, 20000), (206, 200, 3, 100),
+                                            # unequal stage counts per class (the
+                                            # classes share one row pitch): cases the
+                                            # larger class, then the smaller
+                                            (207, 3000, 90, 2200), (208, 2500, 100, 400)])
 @VARIANTS
 def test_mfma_all_scores_bit_exact(epi, orc, ctx, seed, N, M, cases, var):
     say("mfma dump", var, seed, N, M)
@@ -97,3 +101,20 @@ def test_kernels_alternate_on_one_cohort(epi, ctx):
     ref = [(c.i, c.j, c.k, c.score) for c in runs[0].top]
     for r in runs[1:]:
         assert [(c.i, c.j, c.k, c.score) for c in r.top] == ref
+
+
+@VARIANTS
+def test_mfma_cohorts_in_reused_blocks(epi, orc, ctx, var):
+    """Device blocks of destroyed cohorts are kept and handed to later
+    cohorts without being cleared: cohorts created into blocks that held
+    other cohorts (larger, smaller, other class balance) scan exactly."""
+    shapes = [(4000, 200, 2000), (1200, 120, 300), (4000, 200, 3200), (700, 190, 350),
+              (2600, 160, 1300)]
+    for b, (N, M, cases) in enumerate(shapes):
+        g, p = epi.generate_hwe(700 + b, N, M, cases)
+        with ctx.cohort(g, p) as co:
+            r = ctx.scan(co, tile={"variant": var})
+        n, top = orc.PackedCohort(g, p).scan()
+        say("reused blocks", var, N, M, cases, (r.i, r.j, r.k), top[0][1:])
+        assert [(c.i, c.j, c.k) for c in r.top] == [t[1:] for t in top]
+        assert [c.score for c in r.top] == [t[0] for t in top]
