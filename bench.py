@@ -346,6 +346,8 @@ def main():
         batched["frac_of_single_scan"] = batched["value"] / value
     if serving:
         serving["frac_of_single_scan"] = serving["value"] / value
+        # the same stream as one-request-at-a-time plain scans (FIFO)
+        serving["vs_plain_fifo"] = serving["value"] / serving["plain_fifo"]["value"]
     cpu = None
     if not args.no_cpu_baseline:
         try:
