@@ -110,6 +110,10 @@ int epi_device_name(epi_ctx *ctx, char *buf, size_t len);
  * :220-230, :256-274).  n_snps >= 3, n_samples >= 1. */
 int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
                       uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
+/* Waits for the context's queued work, then gives the cohort's device
+ * blocks to the context, which keeps them (up to a quarter of device
+ * memory) for later cohorts and frees them at epi_close, or earlier when
+ * an allocation would otherwise fail. */
 int epi_cohort_destroy(epi_cohort *cohort);
 /* Device bytes held by a cohort (its admission footprint). */
 int epi_cohort_bytes(const epi_cohort *cohort, uint64_t *bytes);
