@@ -200,7 +200,8 @@ def serving_leg(epi, ctx, capacity, seed=5, n=64, load=0.8):
     arrivals = np.cumsum(gaps) - gaps[0]
     # served: submit at the arrival times from a separate thread
     results = [None] * n
-    with ctx.server(max_active=32, max_samples=16384, round_ms=10.0) as srv:
+    round_ms, max_active = 5.0, 32
+    with ctx.server(max_active=max_active, max_samples=16384, round_ms=round_ms) as srv:
         ids = [None] * n
         t0 = time.perf_counter()
 
@@ -234,6 +235,7 @@ def serving_leg(epi, ctx, capacity, seed=5, n=64, load=0.8):
     total = sum(w for _, _, w in reqs)
     ok = all(same_result(a, b) for a, b in zip(plain, results))
     return {"requests": n, "offered_load": load, "arrival_rate_per_s": lam,
+            "round_ms": round_ms, "max_active": max_active,
             "triplets_x_samples": total,
             "value": total / t_end, "makespan_ms": t_end * 1e3,
             "device_busy_frac": st["device_ms"] / (t_end * 1e3),
