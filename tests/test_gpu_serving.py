@@ -133,6 +133,41 @@ def test_server_memory_admission_and_cancel(epi, ctx):
             assert same(ctx.scan(co), r)
 
 
+def test_server_cancel_while_running_and_tight_memory(epi, ctx):
+    """Requests cancelled while their rounds run and a budget that admits
+    only a few cohorts at a time: device blocks are released and reused by
+    later cohorts while other rounds are on the device, and every request
+    that is not cancelled completes with its plain-scan result."""
+    data = make_cohorts(epi, ctx, 14, 23, (60, 400), (200, 6000))
+    plain, one = [], 0
+    for g, p in data:
+        with ctx.cohort(g, p) as co:
+            plain.append(ctx.scan(co))
+            one = max(one, co.device_bytes)
+    # the largest cohort fits (its expansion included); a few run at a time
+    with ctx.server(memory_budget=int(one * 3), max_active=8, round_ms=1.0) as srv:
+        ids = [srv.submit(g, p) for g, p in data]
+        time.sleep(0.02)
+        dropped = set(range(1, len(ids), 4))
+        for b in dropped:
+            srv.cancel(ids[b])
+        got = {}
+        for b, rid in enumerate(ids):
+            try:
+                got[b] = srv.poll(rid, 120000)
+            except epi.EpiError as e:
+                assert b in dropped and e.code == epi.EPI_ECANCELED, (b, e.code)
+                got[b] = None
+        st = srv.stats()
+    say(st)
+    assert st["failed"] == 0 and st["completed"] + st["cancelled"] == len(ids)
+    for b, r in got.items():
+        if r is None:
+            assert b in dropped
+        else:
+            assert same(plain[b], r), b
+
+
 def test_server_poll_unknown_and_timeout(epi, ctx):
     with ctx.server() as srv:
         with pytest.raises(epi.EpiError) as e:
