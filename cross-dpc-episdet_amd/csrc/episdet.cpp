@@ -216,11 +216,15 @@ extern "C" int epi_open(int device, epi_ctx **out)
         ctx->name = std::string("AMD GPU ") + prop.gcnArchName;
     ctx->arch = prop.gcnArchName;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->stage_ev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         delete ctx;
         return EPI_EDEVICE;
     }
+    for (hipEvent_t &e : ctx->stage_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete ctx;
+            return EPI_EDEVICE;
+        }
     tune_load_default(ctx);
     *out = ctx;
     return EPI_OK;
@@ -230,8 +234,9 @@ epi_ctx::~epi_ctx()
 {
     if (batch_graph)
         (void)hipGraphExecDestroy(batch_graph);
-    if (stage_ev)
-        (void)hipEventDestroy(stage_ev);
+    for (hipEvent_t e : stage_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (ev0)
         (void)hipEventDestroy(ev0);
     if (ev1)

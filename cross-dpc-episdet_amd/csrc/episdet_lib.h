@@ -156,9 +156,14 @@ int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant);
 int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
                   const std::pair<size_t, size_t> *ranges, uint32_t n, bool timed,
                   float *dump_scores, uint32_t *dump_counts);
-// Rescoring of the running lists of jobs, copy back, epi_result assembly.
-int finish_jobs(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
-                double scan_ms, double kernel_ms);
+// Enqueues the rescoring of the jobs' running lists and the copy of the
+// rescored candidates into `out` (pinned, grown as needed); once the stream
+// has passed that copy, assemble_finished turns them into epi_results.
+int enqueue_finish(epi_ctx *ctx, Job *const *jobs, uint32_t n, HostBuf &out);
+int assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results);
+// After the context's stream has been synchronized: no staged upload is
+// still in flight.
+void stage_synced(epi_ctx *ctx);
 
 }  // namespace epi
 
@@ -182,9 +187,14 @@ struct epi_ctx {
     // bytes), kept so that creating a cohort frees nothing (a free waits for
     // the device, which may be running a scan of other cohorts)
     epi::DevBuf cgeno, corder, cbed;
-    epi::HostBuf stage;
-    hipEvent_t stage_ev = nullptr;  // marks the last copy out of `stage`
-    bool stage_pending = false;
+    // pinned staging buffers of host -> device uploads, used in turn, so
+    // that an upload queued behind running work does not hold up the next
+    // one (each buffer's event marks the copy out of it)
+    static constexpr int kStageRing = 8;
+    epi::HostBuf stage[kStageRing];
+    hipEvent_t stage_ev[kStageRing] = {};
+    bool stage_pending[kStageRing] = {};
+    uint32_t stage_next = 0;
     // captured batch graph, reused while the batch signature is unchanged
     hipGraphExec_t batch_graph = nullptr;
     std::vector<uint64_t> batch_sig;

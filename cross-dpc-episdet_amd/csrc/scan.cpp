@@ -238,23 +238,29 @@ int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant)
     return EPI_OK;
 }
 
-// Copies `bytes` from host memory to `dst` through the pinned staging
-// buffer.  The staging buffer is reused, so the previous upload must have
-// left it first.
+// Copies `bytes` from host memory to `dst` through the next pinned staging
+// buffer of the ring, after the upload that last used it has left it.
 int stage_upload(epi_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     if (bytes == 0)
         return EPI_OK;
-    if (ctx->stage_pending) {
-        EPI_HIP(hipEventSynchronize(ctx->stage_ev));
-        ctx->stage_pending = false;
+    const uint32_t b = ctx->stage_next++ % epi_ctx::kStageRing;
+    if (ctx->stage_pending[b]) {
+        EPI_HIP(hipEventSynchronize(ctx->stage_ev[b]));
+        ctx->stage_pending[b] = false;
     }
-    EPI_TRY(ctx->stage.reserve(bytes));
-    std::memcpy(ctx->stage.ptr, src, bytes);
-    EPI_HIP(hipMemcpyAsync(dst, ctx->stage.ptr, bytes, hipMemcpyHostToDevice, ctx->stream));
-    EPI_HIP(hipEventRecord(ctx->stage_ev, ctx->stream));
-    ctx->stage_pending = true;
+    EPI_TRY(ctx->stage[b].reserve(bytes));
+    std::memcpy(ctx->stage[b].ptr, src, bytes);
+    EPI_HIP(hipMemcpyAsync(dst, ctx->stage[b].ptr, bytes, hipMemcpyHostToDevice, ctx->stream));
+    EPI_HIP(hipEventRecord(ctx->stage_ev[b], ctx->stream));
+    ctx->stage_pending[b] = true;
     return EPI_OK;
+}
+
+void stage_synced(epi_ctx *ctx)
+{
+    for (bool &p : ctx->stage_pending)
+        p = false;
 }
 
 size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -457,7 +463,7 @@ static int copy_back_and_assemble(epi_ctx *ctx, Job *const *jobs, uint32_t n, ep
     EPI_HIP(hipMemcpyAsync(&overflow, (uint32_t *)ctx->counters.ptr + 1 + n, sizeof(uint32_t),
                            hipMemcpyDeviceToHost, ctx->stream));
     EPI_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->stage_pending = false;
+    stage_synced(ctx);
     const double wall = ms_since(t0);
     float kms = 0.0f;
     if (timed)
@@ -469,18 +475,20 @@ static int copy_back_and_assemble(epi_ctx *ctx, Job *const *jobs, uint32_t n, ep
     return EPI_OK;
 }
 
-int finish_jobs(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
-                double scan_ms, double kernel_ms)
+int enqueue_finish(epi_ctx *ctx, Job *const *jobs, uint32_t n, HostBuf &out)
 {
     EPI_TRY(enqueue_rescore(ctx, jobs, n));
-    std::vector<Rescored> host((size_t)n * kTop);
-    EPI_HIP(hipMemcpyAsync(host.data(), ctx->rescored.ptr, sizeof(Rescored) * host.size(),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    EPI_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->stage_pending = false;
+    const size_t bytes = sizeof(Rescored) * (size_t)n * kTop;
+    EPI_TRY(out.reserve(bytes));
+    EPI_HIP(hipMemcpyAsync(out.ptr, ctx->rescored.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return EPI_OK;
+}
+
+int assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results)
+{
+    const Rescored *rs = (const Rescored *)in;
     for (uint32_t b = 0; b < n; ++b)
-        EPI_TRY(assemble(host.data() + (size_t)b * kTop, jobs[b]->co, results[b], scan_ms,
-                         kernel_ms));
+        EPI_TRY(assemble(rs + (size_t)b * kTop, jobs[b]->co, results[b], 0.0, 0.0));
     return EPI_OK;
 }
 
@@ -634,7 +642,7 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
         EPI_HIP(hipMemcpyAsync(counts, dc.ptr, ntrip * 54 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, ctx->stream));
     EPI_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->stage_pending = false;
+    stage_synced(ctx);
     return EPI_OK;
 }
 
@@ -671,7 +679,7 @@ extern "C" int epi_count(epi_ctx *ctx, epi_cohort *co, const uint32_t *triplets,
     EPI_HIP(hipMemcpyAsync(host.data(), dr.ptr, sizeof(Rescored) * n, hipMemcpyDeviceToHost,
                            ctx->stream));
     EPI_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->stage_pending = false;
+    stage_synced(ctx);
     for (uint32_t t = 0; t < n; ++t) {
         if (counts)
             std::memcpy(counts + 54 * (size_t)t, host[t].counts, sizeof(host[t].counts));

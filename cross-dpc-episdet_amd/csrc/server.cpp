@@ -51,6 +51,7 @@ struct Request {
     uint64_t footprint = 0;
     Clock::time_point submitted, admitted;
     bool cancel = false;
+    bool finishing = false;  // its last rows are in the launched round
     bool done = false;
     bool charged = false;  // footprint counted in bytes_in_use
     int status = EPI_OK;
@@ -89,6 +90,12 @@ struct epi_server {
     hipGraphExec_t graph = nullptr;
     std::vector<uint64_t> graph_sig;
     hipStream_t upload = nullptr;  // cohort creation, overlapping the rounds
+    // requests whose last rows the launched round scans: their rescoring
+    // and the copy of its results into `fin_out` follow the round on the
+    // stream, and `fin_ev` marks the end of both
+    std::vector<Request *> fin;
+    HostBuf fin_out;
+    hipEvent_t fin_ev = nullptr;
 
     void run();
     int round_launch(std::vector<Request *> &act, double &units);
@@ -145,8 +152,8 @@ void epi_server::release_locked_ctx(Request *r)
     r->job.rows.shrink_to_fit();
 }
 
-// Launches one scheduling round over `act` (ctx->mu held); `units` is its
-// work estimate.
+// Launches one scheduling round over `act` (ctx->mu held), followed by the
+// rescoring of the requests it finishes; `units` is its work estimate.
 int epi_server::round_launch(std::vector<Request *> &act, double &units)
 {
     const uint32_t cap = cfg.max_active;
@@ -203,15 +210,25 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
     EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     EPI_HIP(hipGraphLaunch(graph, ctx->stream));
     EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    fin.clear();
+    std::vector<Job *> fj;
+    for (uint32_t b = 0; b < n; ++b)
+        if (act[b]->job.next_row >= act[b]->job.rows.size()) {
+            act[b]->finishing = true;
+            fin.push_back(act[b]);
+            fj.push_back(&act[b]->job);
+        }
+    if (!fj.empty())
+        EPI_TRY(enqueue_finish(ctx, fj.data(), (uint32_t)fj.size(), fin_out));
+    EPI_HIP(hipEventRecord(fin_ev, ctx->stream));
     return EPI_OK;
 }
 
-// Waits for the launched round (ctx->mu held) and refines the throughput
-// estimate from its device time.
+// Waits for the launched round and its rescoring (ctx->mu held) and refines
+// the throughput estimate from the round's device time.
 int epi_server::round_wait(double units)
 {
-    EPI_HIP(hipEventSynchronize(ctx->ev1));
-    ctx->stage_pending = false;
+    EPI_HIP(hipEventSynchronize(fin_ev));
     float dms = 0.0f;
     EPI_HIP(hipEventElapsedTime(&dms, ctx->ev0, ctx->ev1));
     if (dms > 0.05f && units > 0.0) {
@@ -276,7 +293,7 @@ void epi_server::run()
                 if (as != EPI_OK) {
                     (void)hipStreamSynchronize(upload);
                     (void)hipStreamSynchronize(ctx->stream);
-                    ctx->stage_pending = false;
+                    stage_synced(ctx);
                     release_locked_ctx(r);
                     r->status = as;
                     failed.push_back(r);
@@ -297,7 +314,7 @@ void epi_server::run()
                         drop.push_back(r);
                     else if (st != EPI_OK)
                         drop.push_back(r);
-                    else if (r->job.next_row >= r->job.rows.size())
+                    else if (r->finishing)
                         done_ok.push_back(r);
                     else
                         keep.push_back(r);
@@ -306,14 +323,19 @@ void epi_server::run()
                 stats.active = (uint32_t)active.size();
             }
             if (!done_ok.empty()) {
+                // the rescored lists of `fin`, in its order (a request of
+                // it that was cancelled meanwhile is dropped instead)
                 std::vector<Job *> jp;
-                for (Request *r : done_ok)
+                for (Request *r : fin)
                     jp.push_back(&r->job);
-                std::vector<epi_result> res(done_ok.size());
-                const int fs = finish_jobs(ctx, jp.data(), (uint32_t)jp.size(), res.data(), 0.0, 0.0);
+                std::vector<epi_result> res(fin.size());
+                const int fs = assemble_finished(fin_out.ptr, jp.data(), (uint32_t)jp.size(),
+                                                 res.data());
                 const auto now = Clock::now();
-                for (size_t b = 0; b < done_ok.size(); ++b) {
-                    Request *r = done_ok[b];
+                for (size_t b = 0; b < fin.size(); ++b) {
+                    Request *r = fin[b];
+                    if (std::find(done_ok.begin(), done_ok.end(), r) == done_ok.end())
+                        continue;
                     r->result = res[b];
                     r->result.latency_ms = ms_between(r->submitted, now);
                     r->result.queue_ms = ms_between(r->submitted, r->admitted);
@@ -322,8 +344,9 @@ void epi_server::run()
                     finished.push_back(r);
                 }
             }
+            fin.clear();
             (void)hipStreamSynchronize(ctx->stream);
-            ctx->stage_pending = false;
+            stage_synced(ctx);
             for (Request *r : finished)
                 release_locked_ctx(r);
             for (Request *r : drop) {
@@ -372,6 +395,7 @@ extern "C" int epi_server_create(epi_ctx *ctx, const epi_server_config *config,
         s->scfg = resolve_config(ctx, nullptr, cfg.max_samples);
         EPI_TRY(plan_launch(ctx, s->scfg, cfg.max_samples + 2, false, s->plan));
         EPI_HIP(hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking));
+        EPI_HIP(hipEventCreateWithFlags(&s->fin_ev, hipEventDisableTiming));
     }
     s->ctx = ctx;
     s->cfg = cfg;
@@ -403,6 +427,8 @@ extern "C" int epi_server_destroy(epi_server *s)
             (void)hipGraphExecDestroy(s->graph);
         if (s->upload)
             (void)hipStreamDestroy(s->upload);
+        if (s->fin_ev)
+            (void)hipEventDestroy(s->fin_ev);
     }
     delete s;
     return EPI_OK;
