@@ -7,14 +7,16 @@
 //      (sized from the measured throughput to take about round_ms of device
 //      time) as a contiguous range of its remaining tile rows; the ranges
 //      of all requests become the segments of one persistent scan launch,
-//      replayed from a captured HIP graph (queue reset, scan, merge);
+//      replayed from a captured HIP graph (queue reset, scan, merge),
+//      followed on the stream by the rescoring of the requests whose last
+//      rows it scans and the copy of their results to pinned memory;
 //   2. admission, while that round runs on the device: queued requests are
 //      packed into HBM (cohort planes and pair margins on a stream of the
 //      server's own, then the resident row table) in FIFO order while the
 //      device footprint stays within the budget and fewer than max_active
 //      run; they join the next round;
-//   3. after the round, requests whose rows are exhausted are rescored and
-//      completed, and their device memory is released.
+//   3. after the round (one wait), the finished requests complete and
+//      their device memory goes back to the context's pool.
 // Every triplet of a request is scored exactly once whatever the round
 // split, and each round folds the new candidates into the request's exact
 // running top-64, so a result is identical to epi_scan of the same data.
