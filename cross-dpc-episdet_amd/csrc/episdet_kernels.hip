@@ -566,10 +566,12 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-// One workgroup per cohort: start from the cohort's running list, fold in
-// the entries the wavefronts appended (960 at a time, rank of each entry
-// among 1024), write the list back.  Entries not better than the current
-// 64th best are skipped without ranking.
+// One workgroup per cohort: start from the cohort's running list (sorted),
+// fold in the entries the wavefronts appended, 960 at a time after the
+// current best 64, and write the list back.  A pass whose entries all lose
+// to the current 64th best is skipped; otherwise the 1024 entries are
+// sorted by (score, key) with a bitonic network (55 compare-exchange steps,
+// one entry per thread), and the first 64 are the new best.
 __global__ void __launch_bounds__(1024)
 merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
              const uint32_t *__restrict__ counters, uint32_t cap, Cand *const *lists)
@@ -597,23 +599,25 @@ merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
             ent[t] = e < n ? src[e] : empty;
         }
         __syncthreads();
-        const Cand me = ent[t];
         const Cand thr = ent[kTop - 1];
-        uint32_t rank = kTop;
-        if (me.key != kEmptyKey && (t < (uint32_t)kTop || cand_less(me.s, me.key, thr.s, thr.key))) {
-            rank = 0;
-            for (uint32_t f = 0; f < 1024; ++f) {
-                const Cand o = ent[f];
-                rank += cand_less(o.s, o.key, me.s, me.key) ? 1u : 0u;
-            }
+        const Cand me = ent[t];
+        const bool better = t >= (uint32_t)kTop && me.key != kEmptyKey &&
+                            cand_less(me.s, me.key, thr.s, thr.key);
+        if (__syncthreads_or(better)) {
+            for (uint32_t k = 2; k <= 1024; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    const uint32_t q = t ^ j;
+                    if (q > t) {
+                        const Cand a = ent[t], b = ent[q];
+                        const bool up = (t & k) == 0;
+                        if (up ? cand_less(b.s, b.key, a.s, a.key) : cand_less(a.s, a.key, b.s, b.key)) {
+                            ent[t] = b;
+                            ent[q] = a;
+                        }
+                    }
+                    __syncthreads();
+                }
         }
-        __syncthreads();
-        if (t < (uint32_t)kTop)
-            ent[t] = empty;
-        __syncthreads();
-        if (rank < (uint32_t)kTop)
-            ent[rank] = me;
-        __syncthreads();
     }
     if (t < (uint32_t)kTop)
         list[t] = ent[t];
