@@ -566,21 +566,43 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-// One workgroup per cohort: start from the cohort's running list (sorted),
-// fold in the entries the wavefronts appended, 960 at a time after the
-// current best 64, and write the list back.  A pass whose entries all lose
-// to the current 64th best is skipped; otherwise the 1024 entries are
-// sorted by (score, key) with a bitonic network (55 compare-exchange steps,
-// one entry per thread), and the first 64 are the new best.
+// Bitonic sort of the 1024 entries of `ent` by (score, key), one entry per
+// thread of the 1024-thread workgroup (55 compare-exchange steps).
+__device__ void sort1024(Cand *ent, uint32_t t)
+{
+    for (uint32_t k = 2; k <= 1024; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t q = t ^ j;
+            if (q > t) {
+                const Cand a = ent[t], b = ent[q];
+                const bool up = (t & k) == 0;
+                if (up ? cand_less(b.s, b.key, a.s, a.key) : cand_less(a.s, a.key, b.s, b.key)) {
+                    ent[t] = b;
+                    ent[q] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One workgroup per cohort: folds the entries the wavefronts appended into
+// the cohort's running list (sorted, the first 64 of `ent`).  The entries
+// are read 512 at a time (the next 512 loaded while these are filtered);
+// only those better than the current 64th best are appended behind the
+// list, and when the 1024 slots would overflow they are sorted and the best
+// 64 kept.  Most of the tens of thousands of appended entries lose to the
+// 64th best once a few hundred have been seen, so few sorts run.
 __global__ void __launch_bounds__(1024)
 merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
              const uint32_t *__restrict__ counters, uint32_t cap, Cand *const *lists)
 {
     __shared__ Cand ent[1024];
+    __shared__ uint32_t fill;
     const uint32_t cohort = blockIdx.x;
     if (cohort >= hdr->n_cohorts)
         return;
     const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & 63u;
     uint32_t n = counters[1 + cohort];
     if (n > cap)
         n = cap;
@@ -590,35 +612,43 @@ merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
     empty.s = INFINITY;
     empty.pad = 0;
     empty.key = kEmptyKey;
-    if (t < (uint32_t)kTop)
-        ent[t] = list[t];
-    constexpr uint32_t per_pass = 1024 - kTop;
-    for (uint32_t base = 0; base < n; base += per_pass) {
-        if (t >= (uint32_t)kTop) {
-            const uint32_t e = base + (t - kTop);
-            ent[t] = e < n ? src[e] : empty;
+    ent[t] = t < (uint32_t)kTop ? list[t] : empty;
+    if (t == 0)
+        fill = kTop;
+    constexpr uint32_t step = 512;
+    Cand nxt = t < step && t < n ? src[t] : empty;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += step) {
+        const Cand c = nxt;
+        nxt = t < step && base + step + t < n ? src[base + step + t] : empty;
+        const Cand thr = ent[kTop - 1];
+        bool keep = c.key != kEmptyKey && cand_less(c.s, c.key, thr.s, thr.key);
+        const uint32_t cnt = (uint32_t)__syncthreads_count(keep);
+        if (fill + cnt > 1024u) {
+            // make room: the best 64 so far, then filter again against them
+            sort1024(ent, t);
+            if (t >= (uint32_t)kTop)
+                ent[t] = empty;
+            if (t == 0)
+                fill = kTop;
+            __syncthreads();
+            const Cand thr2 = ent[kTop - 1];
+            keep = keep && cand_less(c.s, c.key, thr2.s, thr2.key);
+        }
+        const uint64_t m = __ballot(keep);
+        uint32_t wbase = 0;
+        if (lane == 0 && m)
+            wbase = atomicAdd(&fill, (uint32_t)__popcll(m));
+        wbase = __shfl(wbase, 0);
+        if (keep) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            ent[wbase + below] = c;
         }
         __syncthreads();
-        const Cand thr = ent[kTop - 1];
-        const Cand me = ent[t];
-        const bool better = t >= (uint32_t)kTop && me.key != kEmptyKey &&
-                            cand_less(me.s, me.key, thr.s, thr.key);
-        if (__syncthreads_or(better)) {
-            for (uint32_t k = 2; k <= 1024; k <<= 1)
-                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                    const uint32_t q = t ^ j;
-                    if (q > t) {
-                        const Cand a = ent[t], b = ent[q];
-                        const bool up = (t & k) == 0;
-                        if (up ? cand_less(b.s, b.key, a.s, a.key) : cand_less(a.s, a.key, b.s, b.key)) {
-                            ent[t] = b;
-                            ent[q] = a;
-                        }
-                    }
-                    __syncthreads();
-                }
-        }
     }
+    if (fill > (uint32_t)kTop)
+        sort1024(ent, t);
     if (t < (uint32_t)kTop)
         list[t] = ent[t];
 }
