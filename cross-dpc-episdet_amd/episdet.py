@@ -10,6 +10,7 @@ library or a GPU is missing, calls raise ``EpiError``.
 from __future__ import annotations
 
 import ctypes
+import functools
 import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
@@ -43,6 +44,14 @@ class EpiCandidate(ctypes.Structure):
         ("score_fp64", ctypes.c_double),
         ("counts", _u32 * 54),
     ]
+
+
+# numpy view of EpiCandidate (checked against the ctypes layout at import)
+_CAND_DTYPE = np.dtype([("i", "<u4"), ("j", "<u4"), ("k", "<u4"), ("score", "<f4"),
+                        ("score_fp64", "<f8"), ("counts", "<u4", (54,))])
+assert _CAND_DTYPE.itemsize == ctypes.sizeof(EpiCandidate)
+assert all(_CAND_DTYPE.fields[f][1] == getattr(EpiCandidate, f).offset
+           for f in ("i", "j", "k", "score", "score_fp64", "counts"))
 
 
 class EpiResult(ctypes.Structure):
@@ -265,19 +274,30 @@ class ScanResult:
     queue_ms: float
     max_abs_err_fp32: float
     fp64_order_differs: bool
-    top: List[Candidate] = field(default_factory=list)
+    # the top candidates as a structured array (_CAND_DTYPE), turned into
+    # Candidate objects on first access of `top`
+    cand: np.ndarray = field(default=None, repr=False, compare=False)
 
     @property
     def best(self):
         return (self.i, self.j, self.k)
 
+    @functools.cached_property
+    def top(self) -> List[Candidate]:
+        a = self.cand
+        if a is None:
+            return []
+        counts = a["counts"]
+        return [Candidate(i, j, k, s, s64, counts[e])
+                for e, (i, j, k, s, s64) in enumerate(zip(a["i"].tolist(), a["j"].tolist(),
+                                                          a["k"].tolist(), a["score"].tolist(),
+                                                          a["score_fp64"].tolist()))]
+
     @classmethod
     def from_c(cls, r: EpiResult) -> "ScanResult":
-        top = []
-        for e in range(r.n_top):
-            c = r.top[e]
-            top.append(Candidate(c.i, c.j, c.k, float(c.score), float(c.score_fp64),
-                                 np.ctypeslib.as_array(c.counts).copy()))
+        # one structured copy of the candidates (per-candidate ctypes access
+        # costs microseconds each)
+        top = np.frombuffer(r.top, dtype=_CAND_DTYPE, count=int(r.n_top)).copy()
         return cls(r.i, r.j, r.k, float(r.score), float(r.score_fp64), int(r.triplets),
                    r.n_samples, r.n_snps, r.n_controls, r.n_cases, float(r.scan_ms),
                    float(r.kernel_ms), float(r.latency_ms), float(r.queue_ms),
