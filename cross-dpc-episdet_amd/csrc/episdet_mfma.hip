@@ -305,12 +305,16 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p)
 // 1 KiB block given by M0).  The j and k rows (32 KiB) are 32 copies of 8
 // rows each, 8 per wavefront (wavefronts 0, 1: j rows; 2, 3: k rows); lane L
 // takes row L/8 of its copy and reads the global chunk that the swizzle
-// puts in its slot L%8.  Row placement and swizzle key of SNP s (local to
-// its block) and plane p:
-//   j rows: row 2s + p,            key s & 7
-//   k rows: row 2s + (p ^ (s & 1)), key (s >> 1) & 7
+// puts in its slot L%8.  Row placement and swizzle key of row slot s
+// (0..63) and plane p, and the SNP (local to its block) the slot holds:
+//   j rows: row 2s + p,            key s & 7,        SNP 4 (s % 16) + s / 16
+//   k rows: row 2s + (p ^ (s & 1)), key (s >> 1) & 7, SNP s
 // (the B operand reads of one plane then alternate between the two bank
-// halves; see the operand offsets in the kernel).  SNP i's 256 bytes are one
+// halves; see the operand offsets in the kernel).  Wavefront w reads the j
+// slots 16w..16w+15, which hold SNPs w, w + 4, ..., w + 60: its j spread
+// over the whole block, so the (j, k) pairs past the triangle of a block on
+// the diagonal, or not above SNP i, fall evenly on the four wavefronts and
+// their epilogues take about equal time.  SNP i's 256 bytes are one
 // 4-lane copy per wavefront.  Every wavefront issues kStageCopies + 1
 // copies, so one vmcnt value covers a stage on all.
 //
@@ -330,11 +334,12 @@ __device__ __forceinline__ MfCopyOffsets mf_copy_offsets(uint32_t Lc, uint32_t w
     for (int q = 0; q < kStageCopies; ++q) {
         const uint32_t first = (wave * kStageCopies + q) * 8;
         const uint32_t blk = first >> 7, rowi = (first & 127u) + rsub;
-        const uint32_t sl = rowi >> 1;  // SNP within its block
+        const uint32_t sl = rowi >> 1;  // row slot within its block
         const uint32_t key = blk ? (sl >> 1) & 7u : sl & 7u;
         const uint32_t plane = (rowi & 1u) ^ (blk & sl & 1u);
         const uint32_t chunk = (lane & 7u) ^ key;
-        o.row[q] = (2 * sl + plane) * Lc + 16 * chunk;
+        const uint32_t snp = blk ? sl : 4 * (sl & 15u) + (sl >> 4);
+        o.row[q] = (2 * snp + plane) * Lc + 16 * chunk;
     }
     const uint32_t e = 4 * wave + (lane & 3u);  // 16-byte chunk of SNP i's two rows
     o.irow = (e >> 3) * Lc + 16 * (e & 7u);
@@ -438,13 +443,13 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 
     // Lane-constant LDS offsets of the MFMA operands, in bytes from the
     // start of a stage buffer, per 32-byte step kk (the swizzled slot):
-    //   A (j rows): row l32 of tile fa = (j_local, a, b), j_local =
-    //     16w + 8fa + l32/4, at row 2 j_local + b; tile fa = 1 is 2048 bytes on.
+    //   A (j rows): row l32 of tile fa = (j slot, a, b), j slot =
+    //     16w + 8fa + l32/4, at row 2 slot + b; tile fa = 1 is 2048 bytes on.
     //   B (k rows): column l32 of tile fb = 2 kq + cc is k_local = 32 kq +
     //     l32, plane cc, at row 2 k_local + (cc ^ (k_local & 1)), so a lane
     //     holds both planes of its k, in tiles 2 kq and 2 kq + 1; kq = 1 is
     //     8192 bytes on.
-    //   swizzle keys: A row j_local & 7 = (l32 / 4) & 7; B row (k_local >> 1)
+    //   swizzle keys: A row slot & 7 = (l32 / 4) & 7; B row (k_local >> 1)
     //     & 7 = (l32 / 2) & 7.  16 lanes of one operand read then hit 8 rows
     //     in each bank half (B: alternating parity), at 8 different slots:
     //     no bank conflicts.
@@ -661,7 +666,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 
         // ---- epilogue.  Accumulator element e of tile (fa, fb = 2 kq + cc)
         // in lane (half, l32) is the count of rows (j, ab) x column (k, cc):
-        // j_local = 16w + 8fa + 2 (e/4) + half, ab = e%4, k_local = 32 kq + l32.
+        // j slot 16w + 8fa + 2 (e/4) + half, which holds SNP j_local = 32fa +
+        // 8 (e/4) + 4 half + w, ab = e%4, k_local = 32 kq + l32.
         // So each lane holds all 8 cells of 4 (j, k) pairs per (fa, kq): it
         // scores j-slots s = 0..3 against its own k.  Counts are handled as
         // 4 x counts (byte offsets into the ln(n!) table): the FP4 sums come
@@ -676,15 +682,16 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         for (int t = 0; t < 4; ++t) {
             const int fa = t >> 1, kq = t & 1;
             {
-                // The wavefront's j of this (fa, kq) are [jlo, jlo + 8), its
-                // k [klo, klo + 32): skip it when no i < j < k < M is among
-                // them (SNP i inside the j-block, the lower half of a block
-                // on the diagonal, k-blocks past M).
-                const uint32_t jlo = J0 + 16 * wave + 8 * (uint32_t)fa;
+                // The wavefront's j of this (fa, kq) lie in [jlo, jlo + 28]
+                // (every 4th), its k in [klo, klo + 32): skip it when no
+                // i < j < k < M can be among them (SNP i past the j-range,
+                // the lower half of a block on the diagonal, k-blocks past
+                // M).
+                const uint32_t jlo = J0 + 32 * (uint32_t)fa + wave;
                 const uint32_t klo = K0 + 32 * (uint32_t)kq;
                 const uint32_t jmin = jlo > i + 1 ? jlo : i + 1;
                 const uint32_t khi = klo + 31 < M - 1 ? klo + 31 : M - 1;
-                if (klo >= M || jmin > jlo + 7 || jmin >= khi)
+                if (klo >= M || jmin > jlo + 28 || jmin >= khi)
                     continue;
             }
             uint32_t v[2][2][16];  // [class][cc][e], 4 x counts
@@ -717,7 +724,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             // (j, k) pair margins of the lane's slots, read a slot ahead:
             // row J0 + jl of pair[c], k columns K0 + kl (32 consecutive k per
             // half-wavefront); every (p, q) < Mp is in the table
-            const uint32_t jl0 = 16 * wave + 8 * (uint32_t)fa + half;
+            const uint32_t jl0 = 32 * (uint32_t)fa + 4 * half + wave;  // slot 0's SNP
             const uint4 *jkp[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c)
@@ -733,9 +740,9 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 for (int c = 0; c < 2; ++c) {
                     jkc[c] = x4(jkn[c]);
                     if (sl + 1 < 4)
-                        jkn[c] = jkp[c][(size_t)(2 * sl + 2) * Mp];
+                        jkn[c] = jkp[c][(size_t)(8 * sl + 8) * Mp];
                 }
-                const uint32_t jl = 16 * wave + 8 * (uint32_t)fa + 2 * sl + half;
+                const uint32_t jl = jl0 + 8 * sl;
                 const uint32_t j = J0 + jl;
                 const bool valid = (i < j) && (j < k) && (k < M);
                 // Only lanes with i < j < k are offered; a slot with none
