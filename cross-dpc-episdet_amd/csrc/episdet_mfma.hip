@@ -25,8 +25,10 @@
 // Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
 // 4 wavefronts, one per SIMD.  Per stage the workgroup copies the j-block
 // and k-block rows (and SNP i's) into LDS; wavefront w computes the 64 rows
-// (16 j x 4 (a,b)) of its quarter against all 128 columns (64 k x 2 c):
-// 2 x 4 MFMA tiles of 32 x 32 per class, 256 accumulators in AGPRs.
+// (16 j x 4 (a,b); its j are the block's SNPs w, w + 4, ..., w + 60, so that
+// the pairs outside i < j < k spread evenly over the wavefronts) against all
+// 128 columns (64 k x 2 c): 2 x 4 MFMA tiles of 32 x 32 per class, 256
+// accumulators in AGPRs.
 // Pipelining: two LDS stage buffers, one barrier per stage.  Steps 0 and 1
 // of a stage run before its barrier, steps 2 and 3 after it (operands in
 // registers), and the next stage's copies (global_load_lds, no registers)
