@@ -200,7 +200,7 @@ def serving_leg(epi, ctx, capacity, seed=5, n=64, load=0.8):
     arrivals = np.cumsum(gaps) - gaps[0]
     # served: submit at the arrival times from a separate thread
     results = [None] * n
-    round_ms, max_active = 5.0, 32
+    round_ms, max_active = 3.5, 64
     with ctx.server(max_active=max_active, max_samples=16384, round_ms=round_ms) as srv:
         ids = [None] * n
         t0 = time.perf_counter()

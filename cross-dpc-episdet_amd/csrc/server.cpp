@@ -3,9 +3,9 @@
 // No reference counterpart: the reference scans one generated dataset per
 // process (GPU/v4.cpp:198-624).  Here a worker thread owns the context's
 // stream and runs scheduling rounds:
-//   1. one round: every active request gets a fair share of a work budget
-//      (sized from the measured throughput to take about round_ms of device
-//      time) as a contiguous range of its remaining tile rows; the ranges
+//   1. one round: every active request gets a max-min fair share of a work
+//      budget (sized from the measured throughput to take about round_ms of
+//      device time) as a contiguous range of its remaining tile rows; the ranges
 //      of all requests become the segments of one persistent scan launch,
 //      replayed from a captured HIP graph (queue reset, scan, merge),
 //      followed on the stream by the rescoring of the requests whose last
@@ -160,9 +160,28 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
 {
     const uint32_t cap = cfg.max_active;
     const uint32_t n = (uint32_t)act.size();
-    // fair share of the round budget per request, at least one row each
+    // Max-min fair shares of the round budget: a request that needs less
+    // than an equal share gets what it needs, and what it leaves is split
+    // among the others, so rounds stay full while small requests finish in
+    // their first round.  At least one row each.
     const double budget = std::max(units_per_ms, 1.0) * cfg.round_ms;
-    const double share = budget / std::max<uint32_t>(n, 1);
+    std::vector<double> need(n, 0.0), share(n, 0.0);
+    for (uint32_t b = 0; b < n; ++b) {
+        const Job &J = act[b]->job;
+        for (size_t r = J.next_row; r < J.rows.size(); ++r)
+            need[b] += row_units(J.rows[r], J.nw, J.co);
+    }
+    std::vector<uint32_t> by_need(n);
+    for (uint32_t b = 0; b < n; ++b)
+        by_need[b] = b;
+    std::sort(by_need.begin(), by_need.end(),
+              [&](uint32_t a, uint32_t c) { return need[a] < need[c]; });
+    double left = budget;
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t b = by_need[q];
+        share[b] = std::min(need[b], left / (double)(n - q));
+        left -= share[b];
+    }
     std::vector<Job *> jp(n);
     std::vector<std::pair<size_t, size_t>> ranges(n);
     units = 0.0;
@@ -171,7 +190,8 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
         jp[b] = &J;
         size_t r = J.next_row;
         double u = 0.0;
-        while (r < J.rows.size() && (r == J.next_row || u + row_units(J.rows[r], J.nw, J.co) <= share)) {
+        while (r < J.rows.size() &&
+               (r == J.next_row || u + row_units(J.rows[r], J.nw, J.co) <= share[b])) {
             u += row_units(J.rows[r], J.nw, J.co);
             ++r;
         }
