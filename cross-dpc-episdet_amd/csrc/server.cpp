@@ -59,6 +59,7 @@ struct Request {
     int status = EPI_OK;
     epi_cohort *co = nullptr;
     Job job;
+    std::vector<double> cum;  // work units of the job's rows [0, r), r = 0..rows
     epi_result result;
 };
 
@@ -136,6 +137,12 @@ int epi_server::admit_locked_ctx(Request *r)
                                   &r->co, upload);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg, nullptr);
+    if (st == EPI_OK) {
+        const Job &J = r->job;
+        r->cum.assign(J.rows.size() + 1, 0.0);
+        for (size_t q = 0; q < J.rows.size(); ++q)
+            r->cum[q + 1] = r->cum[q] + row_units(J.rows[q], J.nw, J.co);
+    }
     r->admitted = Clock::now();
     return st;
 }
@@ -152,6 +159,8 @@ void epi_server::release_locked_ctx(Request *r)
     r->job.own_list.release();
     r->job.rows.clear();
     r->job.rows.shrink_to_fit();
+    r->cum.clear();
+    r->cum.shrink_to_fit();
 }
 
 // Launches one scheduling round over `act` (ctx->mu held), followed by the
@@ -166,11 +175,8 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
     // their first round.  At least one row each.
     const double budget = std::max(units_per_ms, 1.0) * cfg.round_ms;
     std::vector<double> need(n, 0.0), share(n, 0.0);
-    for (uint32_t b = 0; b < n; ++b) {
-        const Job &J = act[b]->job;
-        for (size_t r = J.next_row; r < J.rows.size(); ++r)
-            need[b] += row_units(J.rows[r], J.nw, J.co);
-    }
+    for (uint32_t b = 0; b < n; ++b)
+        need[b] = act[b]->cum.back() - act[b]->cum[act[b]->job.next_row];
     std::vector<uint32_t> by_need(n);
     for (uint32_t b = 0; b < n; ++b)
         by_need[b] = b;
@@ -187,16 +193,18 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
     units = 0.0;
     for (uint32_t b = 0; b < n; ++b) {
         Job &J = act[b]->job;
+        const std::vector<double> &cum = act[b]->cum;
         jp[b] = &J;
-        size_t r = J.next_row;
-        double u = 0.0;
-        while (r < J.rows.size() &&
-               (r == J.next_row || u + row_units(J.rows[r], J.nw, J.co) <= share[b])) {
-            u += row_units(J.rows[r], J.nw, J.co);
-            ++r;
+        // the most rows whose units fit the share, at least one
+        const size_t r0 = J.next_row;
+        size_t r = r0;
+        if (r0 < J.rows.size()) {
+            r = (size_t)(std::upper_bound(cum.begin() + (r0 + 1), cum.end(), cum[r0] + share[b]) -
+                         cum.begin()) - 1;
+            r = std::max(r, r0 + 1);
         }
-        ranges[b] = std::make_pair(J.next_row, r);
-        units += u;
+        ranges[b] = std::make_pair(r0, r);
+        units += cum[r] - cum[r0];
     }
     std::vector<unsigned char> host;
     RoundLayout L;
