@@ -10,6 +10,7 @@ namespace epi {
 constexpr int kLanes = 64;     // CDNA wavefront
 constexpr int kTop = 64;       // candidates kept per cohort
 constexpr int kWaveCap = 128;  // per-wavefront LDS candidate buffer
+constexpr int kMfmaWaves = 8;  // wavefronts per workgroup of the MFMA scan kernel (2 per SIMD)
 
 // Candidate: FP32 score and a lexicographic key (i << 42 | j << 21 | k).
 struct Cand {

@@ -500,7 +500,7 @@ static ScanFn pick(const ScanConfig &cfg, bool tl, bool dump)
 bool scan_config_supported(const ScanConfig &cfg)
 {
     if (cfg.variant == 2 || cfg.variant == 3)
-        return cfg.snps_j == 64 && cfg.waves == 4 && cfg.words == 1;
+        return cfg.snps_j == 64 && cfg.waves == kMfmaWaves && cfg.words == 1;
     return pick(cfg, false, false) != nullptr;
 }
 

@@ -198,11 +198,20 @@ struct MfAcc<true> {
 // ------------------------------------------------------------------------
 // the MFMA scan kernel
 // ------------------------------------------------------------------------
-constexpr int kMfWaves = 4;
+// Eight wavefronts per workgroup, two per SIMD.  The matrix cores take this
+// instruction from a single wavefront at about half the rate they reach
+// with two (independent accumulators in both cases), and one wavefront's
+// VALU and LDS work runs beside the other wavefront's MFMAs on their SIMD
+// at almost no cost to them (both measured on MI355X); two wavefronts per
+// SIMD leave each 256 registers, so each holds 128 accumulators: 16 j x 32 k
+// of the 64 x 64 (j, k) tile, both classes.
+constexpr int kMfWaves = kMfmaWaves;
 constexpr int kStage = 128;           // bytes per row per LDS stage (four 32-byte MFMA steps)
 constexpr int kRowBytes = kStage;     // LDS bytes per staged (snp, plane) row
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
+constexpr int kIChunks = 2 * kStage / 16 / kMfWaves;  // 16-byte chunks of SNP i's rows per wave
+static_assert(kStageCopies == 4 && kIChunks == 2, "copy split");
 
 // Staged rows are 128 bytes with no padding; 16-byte slot s of row r holds
 // chunk s ^ ((r >> 1) & 7) of the row.  LDS serves 16 lanes of a 16-byte
@@ -305,20 +314,20 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p)
 // One stage is copied from global memory straight into an LDS stage buffer
 // (global_load_lds_dwordx4: lane L of a copy writes bytes 16L..16L+15 of a
 // 1 KiB block given by M0).  The j and k rows (32 KiB) are 32 copies of 8
-// rows each, 8 per wavefront (wavefronts 0, 1: j rows; 2, 3: k rows); lane L
+// rows each, 4 per wavefront (wavefronts 0-3: j rows; 4-7: k rows); lane L
 // takes row L/8 of its copy and reads the global chunk that the swizzle
 // puts in its slot L%8.  Row placement and swizzle key of row slot s
 // (0..63) and plane p, and the SNP (local to its block) the slot holds:
 //   j rows: row 2s + p,            key s & 7,        SNP 4 (s % 16) + s / 16
 //   k rows: row 2s + (p ^ (s & 1)), key (s >> 1) & 7, SNP s
 // (the B operand reads of one plane then alternate between the two bank
-// halves; see the operand offsets in the kernel).  Wavefront w reads the j
-// slots 16w..16w+15, which hold SNPs w, w + 4, ..., w + 60: its j spread
-// over the whole block, so the (j, k) pairs past the triangle of a block on
-// the diagonal, or not above SNP i, fall evenly on the four wavefronts and
-// their epilogues take about equal time.  SNP i's 256 bytes are one
-// 4-lane copy per wavefront.  Every wavefront issues kStageCopies + 1
-// copies, so one vmcnt value covers a stage on all.
+// halves; see the operand offsets in the kernel).  The j slots 16q..16q+15
+// hold SNPs q, q + 4, ..., q + 60: each wavefront's j spread over the whole
+// block, so the (j, k) pairs past the triangle of a block on the diagonal,
+// or not above SNP i, fall evenly on the wavefronts and their epilogues
+// take about equal time.  SNP i's 256 bytes are one 2-lane copy per
+// wavefront.  Every copy lands before the barrier that ends the stage
+// (s_waitcnt vmcnt(0) on every wavefront).
 //
 // Per-lane source offsets of the copies within the wavefront's block (and
 // of SNP i's rows), for rows of Lc bytes; fixed for a class.
@@ -343,13 +352,14 @@ __device__ __forceinline__ MfCopyOffsets mf_copy_offsets(uint32_t Lc, uint32_t w
         const uint32_t snp = blk ? sl : 4 * (sl & 15u) + (sl >> 4);
         o.row[q] = (2 * snp + plane) * Lc + 16 * chunk;
     }
-    const uint32_t e = 4 * wave + (lane & 3u);  // 16-byte chunk of SNP i's two rows
+    // 16-byte chunk e of SNP i's two rows (plane e / 8, chunk e % 8)
+    const uint32_t e = kIChunks * wave + (lane % (uint32_t)kIChunks);
     o.irow = (e >> 3) * Lc + 16 * (e & 7u);
     return o;
 }
 
 // Copy q of the wavefront's block rows of a stage (src: the block's rows at
-// the stage's byte offset), and the 4-lane copy of SNP i's rows.
+// the stage's byte offset), and the 2-lane copy of SNP i's rows.
 __device__ __forceinline__ void mf_issue_copy(uint32_t stage, const uint8_t *src,
                                               const MfCopyOffsets &o, uint32_t wave, int q)
 {
@@ -360,28 +370,9 @@ __device__ __forceinline__ void mf_issue_icopy(uint32_t stage, const uint8_t *sr
                                                const MfCopyOffsets &o, uint32_t wave,
                                                uint32_t lane)
 {
-    if (lane < 4)
-        lds_copy16(src, o.irow, stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
-}
-
-// Issues the copies of the stage at byte s0 of the rows.  `blk_base` is the
-// wavefront's block (j-block for wavefronts 0, 1, k-block for 2, 3) and
-// `i_base` SNP i's first row; `stage` is the LDS byte address of the
-// MfStage to fill (all wave-uniform).
-__device__ __forceinline__ void mf_issue_stage(uint32_t stage, const uint8_t *blk_base,
-                                               const uint8_t *i_base, uint32_t s0,
-                                               const MfCopyOffsets &o, uint32_t wave,
-                                               uint32_t lane)
-{
-    const uint8_t *b = blk_base + s0;
-#pragma unroll
-    for (int q = 0; q < kStageCopies; ++q) {
-        const uint32_t first = (wave * kStageCopies + q) * 8;
-        lds_copy16(b, o.row[q], stage + first * kRowBytes);  // krows follows jrows
-    }
-    if (lane < 4)
-        lds_copy16(i_base + s0, o.irow,
-                   stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 64 * wave);
+    if (lane < (uint32_t)kIChunks)
+        lds_copy16(src, o.irow,
+                   stage + (uint32_t)__builtin_offsetof(MfStage, irow) + 16 * kIChunks * wave);
 }
 
 // 4 x each of four counts: the epilogue works on 4 x counts, the byte
@@ -408,6 +399,21 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
         G.ij[c][e] = x4(x);
         G.sj[c][e] = make_uint2(y.x << 2, y.y << 2);
     }
+}
+
+// 4 x a count from an accumulator element: the FP4 sums are already 4 x
+// the count (exact integers in FP32), the int8 ones are shifted.
+template <typename T>
+__device__ __forceinline__ uint32_t acc_x4(T v);
+template <>
+__device__ __forceinline__ uint32_t acc_x4<float>(float v)
+{
+    return (uint32_t)v;
+}
+template <>
+__device__ __forceinline__ uint32_t acc_x4<int>(int v)
+{
+    return (uint32_t)v << 2;
 }
 
 template <bool TLDS, bool DUMP, bool FP4>
@@ -442,21 +448,23 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     // lane roles in the MFMA tiles
     const uint32_t half = lane >> 5;   // which 16 bytes of a 32-sample step
     const uint32_t l32 = lane & 31u;   // A row / B column within a tile
+    const uint32_t jq = wave & 3u;     // j slots 16 jq .. 16 jq + 15 (SNPs jq, jq + 4, ...)
+    const uint32_t kq = wave >> 2;     // k SNPs 32 kq .. 32 kq + 31 of the block
 
     // Lane-constant LDS offsets of the MFMA operands, in bytes from the
     // start of a stage buffer, per 32-byte step kk (the swizzled slot):
     //   A (j rows): row l32 of tile fa = (j slot, a, b), j slot =
-    //     16w + 8fa + l32/4, at row 2 slot + b; tile fa = 1 is 2048 bytes on.
-    //   B (k rows): column l32 of tile fb = 2 kq + cc is k_local = 32 kq +
-    //     l32, plane cc, at row 2 k_local + (cc ^ (k_local & 1)), so a lane
-    //     holds both planes of its k, in tiles 2 kq and 2 kq + 1; kq = 1 is
-    //     8192 bytes on.
+    //     16 jq + 8 fa + l32 / 4, at row 2 slot + b; tile fa = 1 is 2048
+    //     bytes on.
+    //   B (k rows): column l32 of tile cc is k_local = 32 kq + l32, plane
+    //     cc, at row 2 k_local + (cc ^ (k_local & 1)), so a lane holds both
+    //     planes of its k.
     //   swizzle keys: A row slot & 7 = (l32 / 4) & 7; B row (k_local >> 1)
     //     & 7 = (l32 / 2) & 7.  16 lanes of one operand read then hit 8 rows
     //     in each bank half (B: alternating parity), at 8 different slots:
     //     no bank conflicts.
     // A stage's reads add the stage buffer's LDS address (wave-uniform) to
-    // these offsets; the fa, kq and kk parts that are constant fold into the
+    // these offsets; the fa and kk parts that are constant fold into the
     // instructions' offset fields.
     const uint32_t a_lane = (l32 >> 1) & 1u;  // g_i plane of this lane's A rows
     const uint32_t jx = (l32 >> 2) & 7u;
@@ -464,11 +472,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     uint32_t aoff[kStage / 32], boff[2][kStage / 32];
     {
         const uint32_t ja = (uint32_t)__builtin_offsetof(MfStage, jrows) +
-                            ((16 * wave + (l32 >> 2)) * 2 + (l32 & 1u)) * kRowBytes;
-        const uint32_t kb0 = (uint32_t)__builtin_offsetof(MfStage, krows) +
-                             (2 * l32 + (l32 & 1u)) * kRowBytes;
-        const uint32_t kb1 = (uint32_t)__builtin_offsetof(MfStage, krows) +
-                             (2 * l32 + (1u ^ (l32 & 1u))) * kRowBytes;
+                            ((16 * jq + (l32 >> 2)) * 2 + (l32 & 1u)) * kRowBytes;
+        const uint32_t kbase = (uint32_t)__builtin_offsetof(MfStage, krows) + 8192 * kq;
+        const uint32_t kb0 = kbase + (2 * l32 + (l32 & 1u)) * kRowBytes;
+        const uint32_t kb1 = kbase + (2 * l32 + (1u ^ (l32 & 1u))) * kRowBytes;
 #pragma unroll
         for (int kk = 0; kk < kStage / 32; ++kk) {
             aoff[kk] = ja + 16 * ((2 * kk + half) ^ jx);
@@ -524,24 +531,25 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         const EPI_CONST CohortDev *C = mf::as_const(P.cohorts) + cohort;
         const uint32_t M = C->M, Mp = C->Mp;
 
-        typename MfAcc<FP4>::type acc[2][2][4];
+        // acc[class][fa][cc]: rows (j slot 16 jq + 8 fa + ..., a, b) x
+        // columns (k = 32 kq + l32, plane cc)
+        typename MfAcc<FP4>::type acc[2][2][2];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int fa = 0; fa < 2; ++fa)
 #pragma unroll
-                for (int fb = 0; fb < 4; ++fb)
+                for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
                     for (int e = 0; e < 16; ++e)
-                        acc[c][fa][fb][e] = 0;
+                        acc[c][fa][cc][e] = 0;
 
-        // Stage loop (see the pipelining note at the top of this file).
-        // Both classes' rows have one pitch (bytes per (snp, plane) row, a
-        // multiple of kStage, zero padded) and class 1's planes follow class
-        // 0's in one allocation; class c has its own stage count (256
-        // samples per stage as nibbles, 128 as bytes).  The last stage of
-        // class 0 copies class 1's first stage (a byte offset jb further),
-        // so class 1 starts without waiting for memory.
+        // Stage loop.  Both classes' rows have one pitch (bytes per (snp,
+        // plane) row, a multiple of kStage, zero padded) and class 1's
+        // planes follow class 0's in one allocation; class c has its own
+        // stage count (256 samples per stage as nibbles, 128 as bytes).  The
+        // last stage of class 0 copies class 1's first stage (a byte offset
+        // jb further), so class 1 starts without waiting for memory.
         uint32_t nst[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
@@ -557,7 +565,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 continue;
             const uint32_t jb = c == 0 && nst[1] > 0 ? (FP4 ? C->F1off : C->E1off) : 0u;
             const MfCopyOffsets co = mf_copy_offsets(Lc, wave, lane);
-            const uint8_t *blk_base = E + (size_t)((wave >> 1) ? K0 : J0) * 2 * Lc;
+            const uint8_t *blk_base = E + (size_t)((wave >> 2) ? K0 : J0) * 2 * Lc;
             const uint8_t *i_base = E + (size_t)i * 2 * Lc;
             if (!(c == 1 && staged)) {
                 // Both buffers are free here: the previous stage loop ended
@@ -565,18 +573,20 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 // reuses them for margins) ended with the barrier before the
                 // next claim.
                 par = 0;
-                mf_issue_stage(stage_lds[0], blk_base, i_base, 0u, co, wave, lane);
+#pragma unroll
+                for (int q = 0; q < kStageCopies; ++q)
+                    mf_issue_copy(stage_lds[0], blk_base, co, wave, q);
+                mf_issue_icopy(stage_lds[0], i_base, co, wave, lane);
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            // Steps 2 and 3 of a stage run after the barrier that ends it
-            // (their operands are in registers by then): 16 MFMAs that keep
-            // the matrix cores busy while the wavefronts meet at the barrier
-            // and while the next stage's copies are issued, one between each
-            // pair of them (a copy that waits for the memory pipeline then
-            // stalls no MFMA).  Steps 0 and 1 run before the barrier, with
-            // the reads of the following step's operands issued ahead.
-            v4i dA[2][2], dB[2][4];  // the deferred steps' A (masked by SNP i) and B
+            // Step 3 of a stage runs after the barrier that ends it (its
+            // operands are in registers by then): 4 MFMAs that keep the
+            // matrix cores busy while the wavefronts meet at the barrier
+            // and while the next stage's copies are issued, one after each
+            // of them.  Steps 0-2 run before the barrier, each with the
+            // reads of the following step's operands issued ahead.
+            v4i dA[2], dB[2];  // the deferred step's A (masked by SNP i) and B
             for (uint32_t st = 0; st < n_st; ++st) {
                 const uint32_t sb = stage_lds[(st + par) & 1];
                 // The next stage's copies go to the buffer the last stage
@@ -586,233 +596,198 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 // copy they land before this stage's barrier.
                 const uint32_t nsb = stage_lds[(st + 1 + par) & 1];
                 const uint32_t ns0 = st + 1 < n_st ? (st + 1) * kStage : (jb ? jb : st * kStage);
-                v4i ei[kStage / 32];
-#pragma unroll
-                for (int kk = 0; kk < kStage / 32; ++kk)
-                    ei[kk] = lds16a(sb + eoff + 32 * kk);
-                v4i aj[kStage / 32][2], bk[kStage / 32][4];
+                v4i ei[kStage / 32], aj[kStage / 32][2], bk[kStage / 32][2];
                 auto read_step = [&](int kk) {
+                    ei[kk] = lds16a(sb + eoff + 32 * kk);
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
                         aj[kk][fa] = lds16a(sb + aoff[kk] + 2048 * fa);
 #pragma unroll
-                    for (int fb = 0; fb < 4; ++fb)
-                        bk[kk][fb] = lds16a(sb + boff[fb & 1][kk] + 8192 * (fb >> 1));
+                    for (int cc = 0; cc < 2; ++cc)
+                        bk[kk][cc] = lds16a(sb + boff[cc][kk]);
                 };
                 read_step(0);
                 if (st > 0) {
-                    // the previous stage's steps 2, 3 with the copies of the next stage
+                    // the previous stage's step 3 with the copies of the next stage
 #pragma unroll
-                    for (int d = 0; d < 2; ++d) {
-                        if (d == 1)
-                            read_step(1);
-#pragma unroll
-                        for (int m = 0; m < 8; ++m) {
-                            __builtin_amdgcn_sched_barrier(0);
-                            acc[c][m & 1][m >> 1] =
-                                mf_mma(dA[d][m & 1], dB[d][m >> 1], acc[c][m & 1][m >> 1]);
-                            __builtin_amdgcn_sched_barrier(0);
-                            const int q = 4 * d + (m >> 1);  // copy after every second MFMA
-                            if (m & 1)
-                                mf_issue_copy(nsb, blk_base + ns0, co, wave, q);
-                            if (d == 0 && m == 0)
-                                mf_issue_icopy(nsb, i_base + ns0, co, wave, lane);
-                        }
+                    for (int m = 0; m < 4; ++m) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        acc[c][m & 1][m >> 1] = mf_mma(dA[m & 1], dB[m >> 1], acc[c][m & 1][m >> 1]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        mf_issue_copy(nsb, blk_base + ns0, co, wave, m);
+                        if (m == 0)
+                            mf_issue_icopy(nsb, i_base + ns0, co, wave, lane);
                     }
                 } else {
 #pragma unroll
                     for (int q = 0; q < kStageCopies; ++q)
                         mf_issue_copy(nsb, blk_base + ns0, co, wave, q);
                     mf_issue_icopy(nsb, i_base + ns0, co, wave, lane);
-                    read_step(1);
                 }
-                // steps 0 and 1; the operands of steps 2 and 3 are read ahead
+                // steps 0-2; step 3's operands are read during step 2
 #pragma unroll
-                for (int kk = 0; kk < 2; ++kk) {
-                    read_step(kk + 2);
+                for (int kk = 0; kk < 3; ++kk) {
+                    read_step(kk + 1);
                     __builtin_amdgcn_sched_barrier(0);
                     v4i A[2];
 #pragma unroll
                     for (int fa = 0; fa < 2; ++fa)
                         A[fa] = aj[kk][fa] & ei[kk];
 #pragma unroll
-                    for (int fb = 0; fb < 4; ++fb)
+                    for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
                         for (int fa = 0; fa < 2; ++fa)
-                            acc[c][fa][fb] = mf_mma(A[fa], bk[kk][fb], acc[c][fa][fb]);
+                            acc[c][fa][cc] = mf_mma(A[fa], bk[kk][cc], acc[c][fa][cc]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
-                for (int d = 0; d < 2; ++d) {
+                for (int fa = 0; fa < 2; ++fa)
+                    dA[fa] = aj[3][fa] & ei[3];
 #pragma unroll
-                    for (int fa = 0; fa < 2; ++fa)
-                        dA[d][fa] = aj[d + 2][fa] & ei[d + 2];
-#pragma unroll
-                    for (int fb = 0; fb < 4; ++fb)
-                        dB[d][fb] = bk[d + 2][fb];
-                }
+                for (int cc = 0; cc < 2; ++cc)
+                    dB[cc] = bk[3][cc];
                 // this buffer is read and the next stage has landed in the other
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
             }
 #pragma unroll
-            for (int d = 0; d < 2; ++d)
+            for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
-                for (int fb = 0; fb < 4; ++fb)
-#pragma unroll
-                    for (int fa = 0; fa < 2; ++fa)
-                        acc[c][fa][fb] = mf_mma(dA[d][fa], dB[d][fb], acc[c][fa][fb]);
+                for (int fa = 0; fa < 2; ++fa)
+                    acc[c][fa][cc] = mf_mma(dA[fa], dB[cc], acc[c][fa][cc]);
             par = jb ? (n_st & 1) : par;
             staged = jb != 0;
         }
 
-        // ---- epilogue.  Accumulator element e of tile (fa, fb = 2 kq + cc)
-        // in lane (half, l32) is the count of rows (j, ab) x column (k, cc):
-        // j slot 16w + 8fa + 2 (e/4) + half, which holds SNP j_local = 32fa +
-        // 8 (e/4) + 4 half + w, ab = e%4, k_local = 32 kq + l32.
-        // So each lane holds all 8 cells of 4 (j, k) pairs per (fa, kq): it
-        // scores j-slots s = 0..3 against its own k.  Counts are handled as
-        // 4 x counts (byte offsets into the ln(n!) table): the FP4 sums come
-        // out that way, the int8 ones are shifted; margins are staged x 4.
-        // One (fa, kq) at a time (not unrolled) keeps the scoring code single.
+        // ---- epilogue.  Accumulator element e of tile (fa, cc) in lane
+        // (half, l32) is the count of rows (j, ab) x column (k, cc): j slot
+        // 16 jq + 8 fa + 2 (e / 4) + half, which holds SNP j_local = 32 fa +
+        // 8 (e / 4) + 4 half + jq, ab = e % 4, k_local = 32 kq + l32.  So
+        // each lane holds all 8 cells of 8 (j, k) pairs: slot s8 = 4 fa + sl
+        // (sl = e / 4) scores j_local = 32 fa + 8 sl + 4 half + jq against
+        // the lane's own k.  Counts are handled as 4 x counts (byte offsets
+        // into the ln(n!) table); margins are staged x 4.  The slot loop is
+        // not unrolled (one copy of the scoring code); a switch reads the
+        // slot's 16 accumulators.
         const uint32_t n04 = C->n[0] << 2, n14 = C->n[1] << 2;
         // the stage buffers are no longer read
         __syncthreads();
-        mf_stage_margins(L.mg, C, i, J0, K0, Mp, tid);
+        if (tid < 256)
+            mf_stage_margins(L.mg, C, i, J0, K0, Mp, tid);
         __syncthreads();
+        const uint32_t kl = 32 * kq + l32;
+        const uint32_t k = K0 + kl;
+        uint4 ik[2];
+        uint2 sk[2], si[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            ik[c] = L.mg.ik[c][kl];
+            sk[c] = L.mg.sk[c][kl];
+            const uint2 s1 = mf::ldc2(C->single[c] + i);
+            si[c] = make_uint2(s1.x << 2, s1.y << 2);
+        }
+        // (j, k) pair margins of the lane's slots, read a slot ahead: row
+        // J0 + j_local of pair[c], column k; every (p, q) < Mp is in the table
+        const uint32_t jl0 = 4 * half + jq;  // slot 0's j_local
+        const uint4 *jkp[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            jkp[c] = C->pair[c] + ((size_t)J0 + jl0) * Mp + k;
+        uint4 jkn[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            jkn[c] = jkp[c][0];
 #pragma unroll 1
-        for (int t = 0; t < 4; ++t) {
-            const int fa = t >> 1, kq = t & 1;
-            {
-                // The wavefront's j of this (fa, kq) lie in [jlo, jlo + 28]
-                // (every 4th), its k in [klo, klo + 32): skip it when no
-                // i < j < k < M can be among them (SNP i past the j-range,
-                // the lower half of a block on the diagonal, k-blocks past
-                // M).
-                const uint32_t jlo = J0 + 32 * (uint32_t)fa + wave;
-                const uint32_t klo = K0 + 32 * (uint32_t)kq;
-                const uint32_t jmin = jlo > i + 1 ? jlo : i + 1;
-                const uint32_t khi = klo + 31 < M - 1 ? klo + 31 : M - 1;
-                if (klo >= M || jmin > jlo + 28 || jmin >= khi)
-                    continue;
-            }
-            uint32_t v[2][2][16];  // [class][cc][e], 4 x counts
-            switch (t) {
-#define EPI_MF_COPY(T, FA, KQ)                                                       \
-    case T:                                                                          \
-        for (int c = 0; c < 2; ++c)                                                  \
-            for (int cc = 0; cc < 2; ++cc)                                           \
-                for (int e = 0; e < 16; ++e)                                         \
-                    v[c][cc][e] = FP4 ? (uint32_t)acc[c][FA][2 * KQ + cc][e]         \
-                                      : (uint32_t)acc[c][FA][2 * KQ + cc][e] << 2;   \
-        break;
-                EPI_MF_COPY(0, 0, 0)
-                EPI_MF_COPY(1, 0, 1)
-                EPI_MF_COPY(2, 1, 0)
-                EPI_MF_COPY(3, 1, 1)
-#undef EPI_MF_COPY
-            }
-            const uint32_t kl = 32 * (uint32_t)kq + l32;
-            const uint32_t k = K0 + kl;
-            uint4 ik[2];
-            uint2 sk[2], si[2];
+        for (uint32_t s8 = 0; s8 < 8; ++s8) {
+            uint4 jkc[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                ik[c] = L.mg.ik[c][kl];
-                sk[c] = L.mg.sk[c][kl];
-                const uint2 s1 = mf::ldc2(C->single[c] + i);
-                si[c] = make_uint2(s1.x << 2, s1.y << 2);
+                jkc[c] = x4(jkn[c]);
+                // slot s8 + 1, at its j_local offset from slot 0
+                if (s8 + 1 < 8)
+                    jkn[c] = jkp[c][(size_t)(32 * ((s8 + 1) >> 2) + 8 * ((s8 + 1) & 3)) * Mp];
             }
-            // (j, k) pair margins of the lane's slots, read a slot ahead:
-            // row J0 + jl of pair[c], k columns K0 + kl (32 consecutive k per
-            // half-wavefront); every (p, q) < Mp is in the table
-            const uint32_t jl0 = 32 * (uint32_t)fa + 4 * half + wave;  // slot 0's SNP
-            const uint4 *jkp[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-                jkp[c] = C->pair[c] + ((size_t)J0 + jl0) * Mp + k;
-            uint4 jkn[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-                jkn[c] = jkp[c][0];
-#pragma unroll
-            for (int sl = 0; sl < 4; ++sl) {
-                uint4 jkc[2];
+            const uint32_t jl = jl0 + 32 * (s8 >> 2) + 8 * (s8 & 3);
+            const uint32_t j = J0 + jl;
+            const bool valid = (i < j) && (j < k) && (k < M);
+            // Only lanes with i < j < k are offered; a slot with none is
+            // skipped by a wave-uniform branch.  The pair table holds every
+            // (p, q), so counts are exact for every lane and every offset
+            // below is at most 4 (N + 1), inside the table.
+            if (__ballot(valid) == 0)
+                continue;
+            uint32_t t8[2][8];  // [class][4a + 2b + c]
+            switch (s8) {
+#define EPI_MF_SLOT(S8)                                                                   \
+    case S8:                                                                              \
+        for (int c = 0; c < 2; ++c)                                                       \
+            for (int ab = 0; ab < 4; ++ab)                                                \
+                for (int cc = 0; cc < 2; ++cc)                                            \
+                    t8[c][2 * ab + cc] = acc_x4(acc[c][(S8) >> 2][cc][4 * ((S8) & 3) + ab]); \
+        break;
+                EPI_MF_SLOT(0)
+                EPI_MF_SLOT(1)
+                EPI_MF_SLOT(2)
+                EPI_MF_SLOT(3)
+                EPI_MF_SLOT(4)
+                EPI_MF_SLOT(5)
+                EPI_MF_SLOT(6)
+                EPI_MF_SLOT(7)
+#undef EPI_MF_SLOT
+            }
+            float sc = 0.0f;
+            {
+                uint32_t x[2][27];
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-                    jkc[c] = x4(jkn[c]);
-                    if (sl + 1 < 4)
-                        jkn[c] = jkp[c][(size_t)(8 * sl + 8) * Mp];
+                    const uint4 ij = L.mg.ij[c][jl];
+                    const uint4 jk = jkc[c];
+                    const uint2 sj = L.mg.sj[c][jl];
+                    const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
+                    const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
+                    const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
+                    const uint32_t asi[2] = {si[c].x, si[c].y};
+                    const uint32_t asj[2] = {sj.x, sj.y};
+                    const uint32_t ask[2] = {sk[c].x, sk[c].y};
+                    derive27(t8[c], aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
                 }
-                const uint32_t jl = jl0 + 8 * sl;
-                const uint32_t j = J0 + jl;
-                const bool valid = (i < j) && (j < k) && (k < M);
-                // Only lanes with i < j < k are offered; a slot with none
-                // is skipped by a wave-uniform branch (a per-lane `if` is
-                // turned into a select by the compiler, which then scores
-                // every slot).  The pair table holds every (p, q), so counts
-                // are exact for every lane and every offset below is at
-                // most 4 (N + 1), inside the table.
-                float sc = 0.0f;
-                if (__ballot(valid) != 0) {
-                    uint32_t x[2][27];
+                if (DUMP && P.dump_counts && valid) {
+                    const uint64_t rk = dump_rank(M, i, j, k);
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        uint32_t t8[8];
-#pragma unroll
-                        for (int ab = 0; ab < 4; ++ab) {
-                            t8[2 * ab + 0] = v[c][0][4 * sl + ab];
-                            t8[2 * ab + 1] = v[c][1][4 * sl + ab];
-                        }
-                        const uint4 ij = L.mg.ij[c][jl];
-                        const uint4 jk = jkc[c];
-                        const uint2 sj = L.mg.sj[c][jl];
-                        const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
-                        const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
-                        const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
-                        const uint32_t asi[2] = {si[c].x, si[c].y};
-                        const uint32_t asj[2] = {sj.x, sj.y};
-                        const uint32_t ask[2] = {sk[c].x, sk[c].y};
-                        derive27(t8, aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
-                    }
-                    if (DUMP && P.dump_counts && valid) {
-                        const uint64_t rk = dump_rank(M, i, j, k);
-#pragma unroll
-                        for (int m = 0; m < 27; ++m) {
-                            P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
-                            P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
-                        }
-                    }
-                    // K2 in the reference order (GPU/v4.cpp:579-583,
-                    // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
-                    // summed over the cells in order from 0 (the first term
-                    // needs no add: no term is -0, so 0 + t == t bit for bit);
-                    // the table reads of 9 cells are issued ahead of their
-                    // ordered sums.
-#pragma unroll
-                    for (int m0 = 0; m0 < 27; m0 += 9) {
-                        float ta[9], tb[9], td[9];
-#pragma unroll
-                        for (int m = 0; m < 9; ++m) {
-                            const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
-                            ta[m] = tab_at(tab + 4, ob + od);
-                            tb[m] = tab_at(tab, ob);
-                            td[m] = tab_at(tab, od);
-                        }
-                        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                        for (int m = 0; m < 9; ++m) {
-                            const float term = (ta[m] - tb[m]) - td[m];
-                            sc = (m0 + m == 0) ? term : sc + term;
-                        }
+                    for (int m = 0; m < 27; ++m) {
+                        P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
+                        P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
                     }
                 }
-                const float s = valid ? fabsf(sc) : INFINITY;
-                if (DUMP && valid)
-                    P.dump_scores[dump_rank(M, i, j, k)] = s;
-                if (!DUMP)
-                    topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
+                // K2 in the reference order (GPU/v4.cpp:579-583,
+                // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
+                // summed over the cells in order from 0 (the first term
+                // needs no add: no term is -0, so 0 + t == t bit for bit);
+                // the table reads of 9 cells are issued ahead of their
+                // ordered sums.
+#pragma unroll
+                for (int m0 = 0; m0 < 27; m0 += 9) {
+                    float ta[9], tb[9], td[9];
+#pragma unroll
+                    for (int m = 0; m < 9; ++m) {
+                        const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
+                        ta[m] = tab_at(tab + 4, ob + od);
+                        tb[m] = tab_at(tab, ob);
+                        td[m] = tab_at(tab, od);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int m = 0; m < 9; ++m) {
+                        const float term = (ta[m] - tb[m]) - td[m];
+                        sc = (m0 + m == 0) ? term : sc + term;
+                    }
+                }
             }
+            const float s = valid ? fabsf(sc) : INFINITY;
+            if (DUMP && valid)
+                P.dump_scores[dump_rank(M, i, j, k)] = s;
+            if (!DUMP)
+                topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
         }
         if (tid == 0)
             L.misc[0] = claimed;
@@ -820,7 +795,6 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         item = mf::rfl(L.misc[0]);
     }
 }
-
 // Static LDS of the instance; a table too large for LDS reports more than
 // any device has, so that the caller places it in global memory.
 size_t mfma_lds_bytes(bool table_lds, uint32_t table_len)

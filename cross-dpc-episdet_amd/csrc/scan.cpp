@@ -93,9 +93,9 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples)
 {
     ScanConfig mfma;
-    mfma.variant = 2;  // MFMA: 64-SNP j- and k-blocks, 4 wavefronts
+    mfma.variant = 2;  // MFMA: 64-SNP j- and k-blocks, kMfmaWaves wavefronts
     mfma.snps_j = 64;
-    mfma.waves = 4;
+    mfma.waves = kMfmaWaves;
     mfma.words = 1;
     if (tile && (tile->variant == 2 || tile->variant == 3)) {
         mfma.variant = (int)tile->variant;

@@ -40,7 +40,8 @@ int tune_load_file(epi_ctx *ctx, const std::string &path)
             e.max_samples = ms;
             e.cfg.variant = got == 4 ? (int)var : 1;
             e.cfg.snps_j = (int)sj;
-            e.cfg.waves = (int)w;
+            // the MFMA kernels have one shape (64 x 64 tiles, kMfmaWaves wavefronts)
+            e.cfg.waves = e.cfg.variant >= 2 ? kMfmaWaves : (int)w;
             e.cfg.words = 1;
             if (scan_config_supported(e.cfg))
                 t.push_back(e);
@@ -92,7 +93,7 @@ extern "C" int epi_tune_set(epi_ctx *ctx, const uint32_t *max_samples, const epi
         const bool mf = tiles[e].variant == 2 || tiles[e].variant == 3;
         x.cfg.variant = mf ? (int)tiles[e].variant : 1;
         x.cfg.snps_j = mf ? 64 : (int)tiles[e].snps_j;
-        x.cfg.waves = mf ? 4 : (int)tiles[e].waves;
+        x.cfg.waves = mf ? kMfmaWaves : (int)tiles[e].waves;
         x.cfg.words = 1;
         if (!scan_config_supported(x.cfg) || (e && max_samples[e] <= max_samples[e - 1]))
             return EPI_EINVAL;
@@ -150,7 +151,7 @@ extern "C" int epi_autotune(epi_ctx *ctx, uint32_t n_samples, uint32_t n_snps, e
         std::memset(&t, 0, sizeof(t));
         t.variant = v;
         t.snps_j = 64;
-        t.waves = 4;
+        t.waves = kMfmaWaves;
         t.words = 1;
         shapes.push_back(t);
     }

@@ -188,7 +188,7 @@ def test_autotune_and_tile_table(epi):
         say("autotune 4096 x 400:", tile, ms)
         assert ms > 0
         if tile["variant"] in (2, 3):
-            assert tile["snps_j"] == 64 and tile["waves"] == 4
+            assert tile["snps_j"] == 64 and tile["waves"] == 8
         else:
             assert tile["variant"] == 1
             assert tile["snps_j"] in (1, 2, 4, 8) and tile["waves"] in (4, 8, 16)
@@ -196,7 +196,7 @@ def test_autotune_and_tile_table(epi):
         assert ctx.tune_get(1000) == {"variant": 1, "snps_j": 2, "waves": 4}
         assert ctx.tune_get(5000) == tile
         ctx.tune_set([(1 << 20, {"variant": 2})])
-        assert ctx.tune_get(5000) == {"variant": 2, "snps_j": 64, "waves": 4}
+        assert ctx.tune_get(5000) == {"variant": 2, "snps_j": 64, "waves": 8}
         g, p = epi.generate_hwe(2, 1500, 100, 700)
         with ctx.cohort(g, p) as co:
             a = ctx.scan(co)  # the table's MFMA entry
