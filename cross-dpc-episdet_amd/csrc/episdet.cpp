@@ -468,6 +468,8 @@ extern "C" int epi_cohort_destroy(epi_cohort *co)
         return EPI_EINVAL;
     epi_ctx *ctx = co->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (co->server_refs != 0)
+        return EPI_EBUSY;  // a server request still scans it
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete co;

@@ -158,9 +158,11 @@ int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
                   float *dump_scores, uint32_t *dump_counts);
 // Enqueues the rescoring of the jobs' running lists and the copy of the
 // rescored candidates into `out` (pinned, grown as needed); once the stream
-// has passed that copy, assemble_finished turns them into epi_results.
+// has passed that copy, assemble_finished turns them into epi_results, with
+// one status per job (a failed consistency check fails only its own job).
 int enqueue_finish(epi_ctx *ctx, Job *const *jobs, uint32_t n, HostBuf &out);
-int assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results);
+void assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results,
+                       int *status);
 // After the context's stream has been synchronized: no staged upload is
 // still in flight.
 void stage_synced(epi_ctx *ctx);
@@ -218,6 +220,9 @@ struct epi_cohort {
     int rows_jp = 0;
     epi::DevBuf drows;
     uint64_t bytes = 0;
+    // server requests queued or running on this cohort (epi_submit_cohort);
+    // changed with ctx->mu held, and the cohort is not destroyed while nonzero
+    uint32_t server_refs = 0;
     epi::CohortDev host_desc() const;
     void use_pool(epi::DevPool *p);
     ~epi_cohort();

@@ -484,12 +484,12 @@ int enqueue_finish(epi_ctx *ctx, Job *const *jobs, uint32_t n, HostBuf &out)
     return EPI_OK;
 }
 
-int assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results)
+void assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results,
+                       int *status)
 {
     const Rescored *rs = (const Rescored *)in;
     for (uint32_t b = 0; b < n; ++b)
-        EPI_TRY(assemble(rs + (size_t)b * kTop, jobs[b]->co, results[b], 0.0, 0.0));
-    return EPI_OK;
+        status[b] = assemble(rs + (size_t)b * kTop, jobs[b]->co, results[b], 0.0, 0.0);
 }
 
 static int check_cohorts(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n)

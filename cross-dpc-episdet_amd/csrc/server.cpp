@@ -36,6 +36,7 @@
 #include "../../include/episdet.h"
 #include "episdet_internal.h"
 #include "episdet_lib.h"
+#include "server_poll.h"
 
 namespace epi {
 
@@ -55,9 +56,17 @@ struct Request {
     bool cancel = false;
     bool finishing = false;  // its last rows are in the launched round
     bool done = false;
+    bool polling = false;  // a thread waits in epi_poll for it
     bool charged = false;  // footprint counted in bytes_in_use
     int status = EPI_OK;
+    // resident: `co` is the caller's cohort (epi_submit_cohort), counted in
+    // its server_refs; otherwise the server creates and frees it
+    bool resident = false;
     epi_cohort *co = nullptr;
+    // resident: the row table the rounds read, a copy of the request's own
+    // (the cohort's cached one is rebuilt when the caller scans it with
+    // another tile shape)
+    DevBuf rows;
     Job job;
     std::vector<double> cum;  // work units of the job's rows [0, r), r = 0..rows
     epi_result result;
@@ -88,6 +97,7 @@ struct epi_server {
     std::deque<Request *> pending;
     std::vector<Request *> active;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    uint32_t pollers = 0;  // threads inside epi_poll
     epi_server_stats stats;
     double units_per_ms = 1e5;  // throughput estimate, refined by every round
     hipGraphExec_t graph = nullptr;
@@ -99,6 +109,9 @@ struct epi_server {
     std::vector<Request *> fin;
     HostBuf fin_out;
     hipEvent_t fin_ev = nullptr;
+    // the launched round's overflow flag (a candidate that found no slot),
+    // copied to pinned memory after the round
+    HostBuf ovf;
 
     void run();
     int round_launch(std::vector<Request *> &act, double &units);
@@ -106,6 +119,8 @@ struct epi_server {
     int admit_locked_ctx(Request *r);
     void release_locked_ctx(Request *r);
     void complete(Request *r, int status);
+    void fail_all_locked_ctx(std::vector<Request *> &admit, int status,
+                             std::vector<Request *> &failed);
 };
 
 void epi_server::complete(Request *r, int status)
@@ -133,10 +148,21 @@ void epi_server::complete(Request *r, int status)
 // ctx->mu held.
 int epi_server::admit_locked_ctx(Request *r)
 {
-    int st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
+    int st = EPI_OK;
+    if (!r->resident)
+        st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
                                   &r->co, upload);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg, nullptr);
+    if (st == EPI_OK && r->resident) {
+        const size_t bytes = sizeof(Row) * std::max<size_t>(r->job.rows.size(), 1);
+        r->rows.pool = &ctx->pool;
+        st = r->rows.reserve(bytes);
+        if (st == EPI_OK)
+            st = stage_upload(ctx, r->rows.ptr, r->job.rows.data(),
+                              sizeof(Row) * r->job.rows.size());
+        r->job.drows = (const Row *)r->rows.ptr;
+    }
     if (st == EPI_OK) {
         const Job &J = r->job;
         r->cum.assign(J.rows.size() + 1, 0.0);
@@ -151,9 +177,13 @@ int epi_server::admit_locked_ctx(Request *r)
 void epi_server::release_locked_ctx(Request *r)
 {
     if (r->co) {
-        delete r->co;
+        if (r->resident)
+            r->co->server_refs--;
+        else
+            delete r->co;
         r->co = nullptr;
     }
+    r->rows.release();
     r->job.drows = nullptr;
     r->job.list = nullptr;
     r->job.own_list.release();
@@ -240,6 +270,9 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
     EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     EPI_HIP(hipGraphLaunch(graph, ctx->stream));
     EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    EPI_TRY(ovf.reserve(sizeof(uint32_t)));
+    EPI_HIP(hipMemcpyAsync(ovf.ptr, (const uint32_t *)ctx->counters.ptr + 1 + cap,
+                           sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     fin.clear();
     std::vector<Job *> fj;
     for (uint32_t b = 0; b < n; ++b)
@@ -255,7 +288,8 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
 }
 
 // Waits for the launched round and its rescoring (ctx->mu held) and refines
-// the throughput estimate from the round's device time.
+// the throughput estimate from the round's device time.  A round in which a
+// candidate found no slot fails (its lists may miss a triplet).
 int epi_server::round_wait(double units)
 {
     EPI_HIP(hipEventSynchronize(fin_ev));
@@ -265,10 +299,33 @@ int epi_server::round_wait(double units)
         const double rate = units / dms;
         units_per_ms = 0.7 * units_per_ms + 0.3 * rate;
     }
-    std::lock_guard<std::mutex> lk(m);
-    stats.rounds++;
-    stats.device_ms += dms;
-    return EPI_OK;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        stats.rounds++;
+        stats.device_ms += dms;
+    }
+    return *(const volatile uint32_t *)ovf.ptr != 0 ? EPI_EINTERNAL : EPI_OK;
+}
+
+// The device cannot be selected: every active request and the ones being
+// admitted fail with `status` (ctx->mu held), so that no poller waits on
+// them; `failed` collects them for completion.
+void epi_server::fail_all_locked_ctx(std::vector<Request *> &admit, int status,
+                                     std::vector<Request *> &failed)
+{
+    std::vector<Request *> act;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        act.swap(active);
+        stats.active = 0;
+    }
+    act.insert(act.end(), admit.begin(), admit.end());
+    admit.clear();
+    for (Request *r : act) {
+        release_locked_ctx(r);
+        r->status = status;
+        failed.push_back(r);
+    }
 }
 
 void epi_server::run()
@@ -298,17 +355,28 @@ void epi_server::run()
             }
             stats.pending = (uint32_t)pending.size();
         }
+        if (!cancelled.empty()) {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            for (Request *r : cancelled)
+                release_locked_ctx(r);  // resident cohorts: drop the reference
+        }
         for (Request *r : cancelled)
             complete(r, EPI_ECANCELED);
 
         std::vector<Request *> finished, failed;
         {
             std::lock_guard<std::mutex> g(ctx->mu);
-            if (hipSetDevice(ctx->device) != hipSuccess)
-                break;
+            const hipError_t de = hipSetDevice(ctx->device);
+            if (de != hipSuccess) {
+                fail_all_locked_ctx(admit, hip_status(de), failed);
+                std::lock_guard<std::mutex> lk(m);
+                // requests still queued fail at the next pass, which admits
+                // them: none is left waiting
+                stats.pending = (uint32_t)pending.size();
+            }
             // the round of the requests already active
             std::vector<Request *> act;
-            {
+            if (de == hipSuccess) {
                 std::lock_guard<std::mutex> lk(m);
                 for (Request *r : active)
                     if (!(r->cancel || stop))
@@ -359,8 +427,9 @@ void epi_server::run()
                 for (Request *r : fin)
                     jp.push_back(&r->job);
                 std::vector<epi_result> res(fin.size());
-                const int fs = assemble_finished(fin_out.ptr, jp.data(), (uint32_t)jp.size(),
-                                                 res.data());
+                std::vector<int> fs(fin.size(), EPI_OK);
+                assemble_finished(fin_out.ptr, jp.data(), (uint32_t)jp.size(), res.data(),
+                                  fs.data());
                 const auto now = Clock::now();
                 for (size_t b = 0; b < fin.size(); ++b) {
                     Request *r = fin[b];
@@ -370,7 +439,7 @@ void epi_server::run()
                     r->result.latency_ms = ms_between(r->submitted, now);
                     r->result.queue_ms = ms_between(r->submitted, r->admitted);
                     r->result.scan_ms = ms_between(r->admitted, now);
-                    r->status = fs;
+                    r->status = fs[b];
                     finished.push_back(r);
                 }
             }
@@ -452,6 +521,12 @@ extern "C" int epi_server_destroy(epi_server *s)
     if (s->worker.joinable())
         s->worker.join();
     {
+        // every request has completed; let the pollers return
+        std::unique_lock<std::mutex> lk(s->m);
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [s] { return s->pollers == 0; });
+    }
+    {
         std::lock_guard<std::mutex> g(s->ctx->mu);
         if (s->graph)
             (void)hipGraphExecDestroy(s->graph);
@@ -501,25 +576,55 @@ extern "C" int epi_submit(epi_server *s, const uint8_t *geno, const uint8_t *phe
     return EPI_OK;
 }
 
+extern "C" int epi_submit_cohort(epi_server *s, epi_cohort *co, uint64_t *request_id)
+{
+    if (!s || !co || !request_id || co->ctx != s->ctx || co->N > s->cfg.max_samples)
+        return EPI_EINVAL;
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r)
+        return EPI_ENOMEM;
+    r->N = co->N;
+    r->M = co->M;
+    r->resident = true;
+    r->co = co;
+    // device bytes the request adds: its row table copy and running list
+    const uint64_t blocks = (uint64_t)(co->Mp / 64 + 1);
+    r->footprint = blocks * blocks * 8 * sizeof(Row) + sizeof(Cand) * kTop;
+    r->submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        co->server_refs++;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        if (!s->stop) {
+            r->id = s->next_id++;
+            *request_id = r->id;
+            s->pending.push_back(r.get());
+            s->stats.submitted++;
+            s->stats.pending = (uint32_t)s->pending.size();
+            s->all.emplace(r->id, std::move(r));
+        }
+    }
+    if (r) {  // not queued: the server is stopping
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        co->server_refs--;
+        return EPI_EBUSY;
+    }
+    s->cv_work.notify_all();
+    return EPI_OK;
+}
+
 extern "C" int epi_poll(epi_server *s, uint64_t id, int32_t timeout_ms, epi_result *result)
 {
     if (!s)
         return EPI_EINVAL;
     std::unique_lock<std::mutex> lk(s->m);
-    auto it = s->all.find(id);
-    if (it == s->all.end())
-        return EPI_ENOENT;
-    Request *r = it->second.get();
-    auto ready = [&] { return r->done; };
-    if (timeout_ms < 0)
-        s->cv_done.wait(lk, ready);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
-        return EPI_EAGAIN;
-    const int st = r->status;
-    if (st == EPI_OK && result)
-        *result = r->result;
-    s->all.erase(it);
-    return st;
+    return poll_request(s->all, id, lk, s->cv_done, timeout_ms, s->pollers,
+                        [result](const Request &r) {
+                            if (r.status == EPI_OK && result)
+                                *result = r.result;
+                        });
 }
 
 extern "C" int epi_cancel(epi_server *s, uint64_t id)

@@ -114,6 +114,7 @@ _PROTOS = [
     ("epi_server_destroy", ctypes.c_int, [_VP]),
     ("epi_submit", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(EpiResult)]),
+    ("epi_submit_cohort", ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_cancel", ctypes.c_int, [_VP, ctypes.c_uint64]),
     ("epi_server_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(EpiServerStats)]),
     ("epi_cohort_create_bed", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(_VP)]),
@@ -435,8 +436,10 @@ class Cohort:
         return int(b.value)
 
     def close(self) -> None:
+        """Frees the cohort; raises EpiError(EPI_EBUSY) and keeps it while a
+        server request on it has not completed."""
         if self.handle:
-            lib().epi_cohort_destroy(self.handle)
+            _check(lib().epi_cohort_destroy(self.handle), "epi_cohort_destroy")
             self.handle = None
 
     def __enter__(self):
@@ -463,6 +466,14 @@ class Server:
         rid = ctypes.c_uint64()
         _check(lib().epi_submit(self.handle, _ptr(geno), _ptr(pheno), geno.shape[0], geno.shape[1],
                                 ctypes.byref(rid)), "epi_submit")
+        return int(rid.value)
+
+    def submit_cohort(self, cohort: "Cohort") -> int:
+        """Queues a scan of a cohort already on the device (no upload or
+        packing); the cohort cannot be closed until the request completes."""
+        rid = ctypes.c_uint64()
+        _check(lib().epi_submit_cohort(self.handle, cohort.handle, ctypes.byref(rid)),
+               "epi_submit_cohort")
         return int(rid.value)
 
     def poll(self, rid: int, timeout_ms: int = -1) -> Optional[ScanResult]:

@@ -40,7 +40,7 @@ extern "C" {
 #define EPI_EINVAL (-1)     /* bad argument or shape */
 #define EPI_ENOMEM (-2)     /* host or device allocation failed */
 #define EPI_EDEVICE (-3)    /* HIP runtime or kernel error */
-#define EPI_EBUSY (-4)      /* server full / context busy */
+#define EPI_EBUSY (-4)      /* server stopping / request or cohort in use */
 #define EPI_ECANCELED (-5)  /* request was cancelled */
 #define EPI_EAGAIN (-6)     /* result not ready yet (epi_poll) */
 #define EPI_ENOENT (-7)     /* unknown request id */
@@ -113,7 +113,8 @@ int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
 /* Waits for the context's queued work, then gives the cohort's device
  * blocks to the context, which keeps them (up to a quarter of device
  * memory) for later cohorts and frees them at epi_close, or earlier when
- * an allocation would otherwise fail. */
+ * an allocation would otherwise fail.  EPI_EBUSY (nothing freed): a server
+ * request on the cohort has not completed (epi_submit_cohort). */
 int epi_cohort_destroy(epi_cohort *cohort);
 /* Device bytes held by a cohort (its admission footprint). */
 int epi_cohort_bytes(const epi_cohort *cohort, uint64_t *bytes);
@@ -180,14 +181,23 @@ typedef struct {
 } epi_server_stats;
 
 int epi_server_create(epi_ctx *ctx, const epi_server_config *config, epi_server **out);
-/* Cancels what is still queued or running, then stops the request loop. */
+/* Cancels what is still queued or running, waits for the threads inside
+ * epi_poll to return, then stops the request loop. */
 int epi_server_destroy(epi_server *server);
 /* Copies the inputs (they may be freed on return) and queues a scan. */
 int epi_submit(epi_server *server, const uint8_t *geno, const uint8_t *pheno,
                uint32_t n_samples, uint32_t n_snps, uint64_t *request_id);
+/* Queues a scan of a cohort already resident on the server's context
+ * (epi_cohort_create*): nothing is uploaded, packed or recomputed.  The
+ * cohort stays the caller's; epi_cohort_destroy returns EPI_EBUSY until
+ * every request on it has completed.  The result equals epi_scan of the
+ * cohort bit for bit.  Like epi_submit, no reference counterpart. */
+int epi_submit_cohort(epi_server *server, epi_cohort *cohort, uint64_t *request_id);
 /* Waits up to timeout_ms (< 0: forever) for a request.  On success the
  * result is copied out and the request is forgotten.  EPI_EAGAIN: still
- * running; EPI_ECANCELED / EPI_ENOMEM / other codes: the request failed. */
+ * running; EPI_EBUSY: another thread is polling the same request;
+ * EPI_ENOENT: unknown or already collected; EPI_ECANCELED / EPI_ENOMEM /
+ * other codes: the request failed (and is forgotten). */
 int epi_poll(epi_server *server, uint64_t request_id, int32_t timeout_ms,
              epi_result *result);
 int epi_cancel(epi_server *server, uint64_t request_id);

@@ -181,6 +181,71 @@ def test_server_poll_unknown_and_timeout(epi, ctx):
         assert r is not None and r.n_snps == 800
 
 
+def test_server_two_pollers_one_request(epi, ctx):
+    """Two threads poll the same request: one gets the result, the other
+    EPI_EBUSY while the first waits and EPI_ENOENT once it is collected."""
+    import threading
+    g, p = epi.generate_hwe(31, 4096, 700, 2048)
+    with ctx.cohort(g, p) as co:
+        plain = ctx.scan(co)
+    with ctx.server() as srv:
+        for rep in range(3):
+            rid = srv.submit(g, p)
+            got, codes = [], []
+
+            def poller():
+                while True:
+                    try:
+                        r = srv.poll(rid, 5)
+                    except epi.EpiError as e:
+                        if e.code == epi.EPI_EBUSY:
+                            continue
+                        codes.append(e.code)
+                        return
+                    if r is not None:
+                        got.append(r)
+                        return
+
+            th = [threading.Thread(target=poller) for _ in range(3)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join(120)
+            assert len(got) == 1 and codes == [epi.EPI_ENOENT] * 2, (rep, len(got), codes)
+            assert same(plain, got[0])
+
+
+def test_submit_resident_cohort(epi, ctx):
+    """epi_submit_cohort scans cohorts already on the device: results equal
+    epi_scan, the cohort cannot be freed while its request runs, and the same
+    cohort may be queued several times and scanned between requests with
+    another tile shape."""
+    # cohort 0 takes about a hundred milliseconds, so its requests are
+    # still queued or running when the test tries to free it
+    data = [epi.generate_hwe(40, 8192, 1200, 4096)] + make_cohorts(epi, ctx, 5, 41, (30, 500),
+                                                                     (100, 5000))
+    cos = [ctx.cohort(g, p) for g, p in data]
+    try:
+        plain = [ctx.scan(c) for c in cos]
+        with ctx.server() as srv:
+            ids = [srv.submit_cohort(c) for c in cos] + [srv.submit_cohort(cos[0])]
+            # still queued or running: the cohort is held
+            with pytest.raises(epi.EpiError) as e:
+                cos[0].close()
+            assert e.value.code == epi.EPI_EBUSY and cos[0].handle
+            # a plain scan of a held cohort with the VALU tile meanwhile
+            assert same(plain[1], ctx.scan(cos[1], tile={"variant": 1}))
+            res = [srv.poll(rid, 120000) for rid in ids]
+            assert all(same(a, b) for a, b in zip(plain, res[:-1]))
+            assert same(plain[0], res[-1])
+            st = srv.stats()
+            assert st["completed"] == len(ids) and st["failed"] == 0
+        # every request collected: the cohorts can go
+    finally:
+        for c in cos:
+            c.close()
+
+
 def test_autotune_and_tile_table(epi):
     # a context of its own: the tile table is per context and is replaced here
     with epi.Context(0) as ctx:
