@@ -225,6 +225,11 @@ extern "C" int epi_open(int device, epi_ctx **out)
             delete ctx;
             return EPI_EDEVICE;
         }
+    const int ns = noalloc_init(device);
+    if (ns != EPI_OK) {
+        delete ctx;
+        return ns;
+    }
     tune_load_default(ctx);
     *out = ctx;
     return EPI_OK;

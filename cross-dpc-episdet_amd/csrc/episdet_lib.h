@@ -136,10 +136,20 @@ int stage_upload(epi_ctx *ctx, void *dst, const void *src, size_t bytes);
 int build_round(Job *const *jobs, const std::pair<size_t, size_t> *ranges, uint32_t n,
                 uint32_t cap, std::vector<unsigned char> &host, RoundLayout &L, uint64_t &items);
 int reserve_round(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, size_t bytes);
-// Queue reset, scan kernel and merge over the tables in ctx->round (fixed
-// device pointers only, so the sequence can be captured into a graph).
+// Device buffers of one round's kernels: the context's own (ctx_round_bufs)
+// or a caller's workspace.
+struct RoundBufs {
+    void *round = nullptr;         // [hdr][segments][descs][lists]
+    uint32_t *counters = nullptr;  // 2 + 2n words
+    Cand *slots = nullptr;         // n x plan.cap candidates
+    const float *t32 = nullptr;    // ln(n!) table, plan.table_len entries
+};
+RoundBufs ctx_round_bufs(epi_ctx *ctx);
+// Queue reset, scan kernel and merge over the round tables in B.round
+// (fixed device pointers only, so the sequence can be captured into a graph).
 int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, const RoundLayout &L,
-                          bool any_items, bool timed, float *dump_scores, uint32_t *dump_counts);
+                          const RoundBufs &B, bool any_items, bool timed, float *dump_scores,
+                          uint32_t *dump_counts);
 int plan_launch(epi_ctx *ctx, const ScanConfig &cfg, uint32_t table_len, bool dump,
                 LaunchPlan &plan);
 // Prepares a job: the cohort's cached rows for the tile width, its expanded
@@ -163,9 +173,16 @@ int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
 int enqueue_finish(epi_ctx *ctx, Job *const *jobs, uint32_t n, HostBuf &out);
 void assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result *results,
                        int *status);
+// One cohort's epi_result from its kTop rescored candidates (EPI_EINTERNAL
+// if a scan score differs from the explicit recount).
+int assemble(const Rescored *rs, const epi_cohort *co, epi_result &R, double scan_ms,
+             double kernel_ms);
 // After the context's stream has been synchronized: no staged upload is
 // still in flight.
 void stage_synced(epi_ctx *ctx);
+// With `device` current: takes the device address of the static workspace
+// of epi_scan_noalloc (once per device; called by epi_open).
+int noalloc_init(int device);
 
 }  // namespace epi
 

@@ -329,14 +329,25 @@ int reserve_round(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, size_t bytes
     return EPI_OK;
 }
 
-// Queue reset, scan kernel and merge over the tables in ctx->round.  Only
-// device pointers that stay fixed for a given buffer set are passed, so the
-// sequence can be captured into a graph.
-int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, const RoundLayout &L,
-                          bool any_items, bool timed, float *dump_scores, uint32_t *dump_counts)
+RoundBufs ctx_round_bufs(epi_ctx *ctx)
 {
-    unsigned char *base = (unsigned char *)ctx->round.ptr;
-    uint32_t *counters = (uint32_t *)ctx->counters.ptr;
+    RoundBufs B;
+    B.round = ctx->round.ptr;
+    B.counters = (uint32_t *)ctx->counters.ptr;
+    B.slots = (Cand *)ctx->slots.ptr;
+    B.t32 = (const float *)ctx->t32.ptr;
+    return B;
+}
+
+// Queue reset, scan kernel and merge over the round tables in B.round.
+// Only device pointers that stay fixed for a given buffer set are passed,
+// so the sequence can be captured into a graph.
+int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, const RoundLayout &L,
+                          const RoundBufs &B, bool any_items, bool timed, float *dump_scores,
+                          uint32_t *dump_counts)
+{
+    unsigned char *base = (unsigned char *)B.round;
+    uint32_t *counters = B.counters;
     EPI_HIP(launch_fill_u32(counters, 0u, 2 + (size_t)n, ctx->stream));
     ScanParams P;
     std::memset(&P, 0, sizeof(P));
@@ -345,10 +356,10 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
     P.segs = (const Segment *)(base + L.segs);
     P.counters = counters;
     P.thr = counters + 2 + n;
-    P.table = (const float *)ctx->t32.ptr;
+    P.table = B.t32;
     P.table_len = plan.table_len;
     P.slot_cap = plan.cap;
-    P.slots = (Cand *)ctx->slots.ptr;
+    P.slots = B.slots;
     P.overflow = counters + 1 + n;
     P.dump_scores = dump_scores;
     P.dump_counts = dump_counts;
@@ -375,7 +386,8 @@ int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
     EPI_TRY(build_round(jobs, ranges, n, n, host, L, items));
     EPI_TRY(reserve_round(ctx, plan, n, L.total));
     EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
-    return enqueue_round_kernels(ctx, plan, n, L, items > 0, timed, dump_scores, dump_counts);
+    return enqueue_round_kernels(ctx, plan, n, L, ctx_round_bufs(ctx), items > 0, timed,
+                                 dump_scores, dump_counts);
 }
 
 // Rescoring tables [descs][lists] of the jobs, in ctx->fin.
@@ -409,7 +421,7 @@ static int enqueue_rescore(epi_ctx *ctx, Job *const *jobs, uint32_t n)
     return enqueue_rescore_kernel(ctx, n);
 }
 
-static int assemble(const Rescored *rs, const epi_cohort *co, epi_result &R, double scan_ms,
+int assemble(const Rescored *rs, const epi_cohort *co, epi_result &R, double scan_ms,
                     double kernel_ms)
 {
     std::memset(&R, 0, sizeof(R));
@@ -588,7 +600,8 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
         }
         hipGraph_t g = nullptr;
         EPI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_round_kernels(ctx, plan, n, L, items > 0, false, nullptr, nullptr);
+        int st = enqueue_round_kernels(ctx, plan, n, L, ctx_round_bufs(ctx), items > 0, false,
+                                       nullptr, nullptr);
         if (st == EPI_OK)
             st = enqueue_rescore_kernel(ctx, n);
         hipError_t ce = hipStreamEndCapture(ctx->stream, &g);

@@ -254,7 +254,8 @@ int epi_server::round_launch(std::vector<Request *> &act, double &units)
         }
         hipGraph_t g = nullptr;
         EPI_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_round_kernels(ctx, plan, cap, L, true, false, nullptr, nullptr);
+        int st = enqueue_round_kernels(ctx, plan, cap, L, ctx_round_bufs(ctx), true, false,
+                                       nullptr, nullptr);
         hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
         if (st != EPI_OK) {
             if (g)

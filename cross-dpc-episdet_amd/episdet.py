@@ -114,6 +114,8 @@ _PROTOS = [
     ("epi_server_destroy", ctypes.c_int, [_VP]),
     ("epi_submit", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(EpiResult)]),
+    ("epi_scan_noalloc", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile),
+                                        ctypes.POINTER(EpiResult)]),
     ("epi_submit_cohort", ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_cancel", ctypes.c_int, [_VP, ctypes.c_uint64]),
     ("epi_server_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(EpiServerStats)]),
@@ -343,6 +345,15 @@ class Context:
         t = _tile(tile)
         _check(lib().epi_scan(self.handle, cohort.handle, ctypes.byref(t) if t else None,
                               ctypes.byref(r)), "epi_scan")
+        return ScanResult.from_c(r)
+
+    def scan_noalloc(self, cohort: "Cohort", tile=None) -> ScanResult:
+        """epi_scan_noalloc: the same scan with no device allocation (a
+        static workspace per GPU)."""
+        r = EpiResult()
+        t = _tile(tile)
+        _check(lib().epi_scan_noalloc(self.handle, cohort.handle, ctypes.byref(t) if t else None,
+                                      ctypes.byref(r)), "epi_scan_noalloc")
         return ScanResult.from_c(r)
 
     def scan_batch(self, cohorts: Sequence["Cohort"], tile=None) -> List[ScanResult]:

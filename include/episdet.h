@@ -126,6 +126,22 @@ int epi_cohort_bytes(const epi_cohort *cohort, uint64_t *bytes);
 int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
              epi_result *result);
 
+/* epi_scan that allocates no device memory, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Every device
+ * buffer beyond the cohort and the result (row table, round tables,
+ * counters, candidate slots, ln(n!) tables, rescoring) comes from one
+ * static 16 MiB workspace per GPU; a scan needing more returns EPI_EINVAL
+ * (about 4 MiB of candidate slots, 12 bytes per sample and 20 bytes per
+ * triplet-tile row: roughly n_snps <= 8192 with the VALU tile, more with
+ * MFMA tiles).  An MFMA tile is used only when the cohort already holds
+ * that variant's expanded planes (an earlier epi_scan with it built them);
+ * otherwise the VALU kernel runs.  The result equals epi_scan's bit for
+ * bit.  Calls on one GPU share the workspace: calls that overlap in time
+ * race. */
+int epi_scan_noalloc(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
+                     epi_result *result);
+
 /* Many cohorts in one persistent launch over a shared (cohort, tile) work
  * queue; results[b] is identical to epi_scan of cohorts[b]. */
 int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n,
