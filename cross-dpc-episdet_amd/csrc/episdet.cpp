@@ -206,6 +206,8 @@ extern "C" int epi_open(int device, epi_ctx **out)
     ctx->cu_count = prop.multiProcessorCount;
     // destroyed cohorts' blocks are kept up to a quarter of the device
     ctx->pool.limit = prop.totalGlobalMem / 4;
+    ctx->pair_budget = prop.totalGlobalMem / 8;
+    ctx->pair_budget_default = ctx->pair_budget;
     ctx->lds_per_block = prop.sharedMemPerBlock;
     char name[256] = {0};
     if (hipDeviceGetName(name, (int)sizeof(name) - 1, device) == hipSuccess && name[0])
@@ -289,14 +291,20 @@ void epi_cohort::use_pool(epi::DevPool *p)
     for (int c = 0; c < 2; ++c) {
         sm[c].pool = wm[c].pool = pair[c].pool = single[c].pool = p;
     }
-    E.pool = F.pool = desc.pool = drows.pool = p;
+    E.pool = F.pool = desc.pool = drows.pool = pslot.pool = todo.pool = p;
 }
 
 namespace epi {
 
+// Shapes every kernel can launch and count exactly: SNP indices fit the 21
+// bits of a candidate key; the FP4 matrix sums are 4 x a class count in
+// FP32, exact below 2^24, so fewer than 2^22 samples; and the genotype
+// bytes (staged on the device) stay within 2^38, which keeps every
+// one-dimensional launch grid below 2^31 blocks.
 bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps)
 {
-    return n_samples > 0 && n_snps >= 3 && n_snps <= (1u << 21) - 1;
+    return n_samples > 0 && n_samples < kMaxSamples && n_snps >= 3 && n_snps <= kMaxSnps &&
+           (uint64_t)n_samples * n_snps <= (1ull << 38);
 }
 
 // Device bytes a cohort of this shape may hold when scanned by `variant`
@@ -324,7 +332,7 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant)
 // samples with a missing phenotype.
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out, hipStream_t strm)
+                         epi_cohort **out, hipStream_t strm, bool full_pairs)
 {
     // class split in sample order (GPU/v4.cpp:135-171): phenotype 1 = case
     std::vector<uint32_t> order;
@@ -363,18 +371,39 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
         st = dorder.reserve((size_t)kept * 4);
     if (st == EPI_OK && bed)
         st = dbed.reserve(bbytes);
+    // pair-margin strips: all of them within the context's budget, else as
+    // many slots as the budget holds (at least two: a scan round needs the
+    // strips of one j block and one k block)
+    const uint32_t nb = co->Mp / 64;
+    const uint64_t strip_bytes = 2ull * co->Mp * 64 * sizeof(uint4);  // both classes
+    co->nslots = nb;
+    if (!full_pairs && strip_bytes * nb > ctx->pair_budget) {
+        const uint64_t fit = ctx->pair_budget / strip_bytes;
+        co->nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(fit, 2), nb);
+    }
+    co->blocked = co->nslots < nb;
+    co->slot_block.assign(co->nslots, co->blocked ? 0xFFFFFFFFu : 0u);
+    std::vector<uint32_t> map(nb, 0);  // blocked: no block resident yet (slot 0 keeps reads in bounds)
+    for (uint32_t g = 0; g < nb && !co->blocked; ++g) {
+        map[g] = g;
+        co->slot_block[g] = g;
+    }
     for (int c = 0; c < 2 && st == EPI_OK; ++c) {
         const size_t pb = (size_t)std::max<uint32_t>(co->W[c], 1) * co->Mp * sizeof(uint4);
         st = co->sm[c].reserve(pb);
         if (st == EPI_OK)
             st = co->wm[c].reserve(pb);
         if (st == EPI_OK)
-            st = co->pair[c].reserve((size_t)co->Mp * co->Mp * sizeof(uint4));
+            st = co->pair[c].reserve((size_t)co->nslots * co->Mp * 64 * sizeof(uint4));
         if (st == EPI_OK)
             st = co->single[c].reserve((size_t)co->Mp * sizeof(uint2));
     }
     if (st == EPI_OK)
         st = co->desc.reserve(sizeof(CohortDev));
+    if (st == EPI_OK)
+        st = co->pslot.reserve((size_t)nb * sizeof(uint32_t));
+    if (st == EPI_OK && co->blocked)
+        st = co->todo.reserve((size_t)co->nslots * sizeof(uint2));
     hipError_t e = hipSuccess;
     if (st == EPI_OK && bed) {
         e = hipMemcpyAsync(dbed.ptr, bed, bbytes, hipMemcpyHostToDevice, strm);
@@ -396,10 +425,15 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     CohortDev h = co->host_desc();
     if (st == EPI_OK && e == hipSuccess)
         e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm);
-    // the pair and single margins are part of the resident cohort: every
-    // scan reads them, none recomputes them
     if (st == EPI_OK && e == hipSuccess)
-        e = launch_pairs((const CohortDev *)co->desc.ptr, h, strm);
+        e = hipMemcpyAsync(co->pslot.ptr, map.data(), (size_t)nb * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, strm);
+    // the single margins and (within the budget) every pair strip are part
+    // of the resident cohort: every scan reads them, none recomputes them
+    if (st == EPI_OK && e == hipSuccess)
+        e = launch_singles((const CohortDev *)co->desc.ptr, co->Mp, strm);
+    if (st == EPI_OK && e == hipSuccess && !co->blocked)
+        e = launch_strips((const CohortDev *)co->desc.ptr, nullptr, nb, co->Mp, strm);
     if (st == EPI_OK && e == hipSuccess)
         e = hipStreamSynchronize(strm);
     if (st == EPI_OK && e != hipSuccess)
@@ -412,7 +446,7 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     co->bytes = 0;
     for (int c = 0; c < 2; ++c)
         co->bytes += co->sm[c].bytes + co->wm[c].bytes + co->pair[c].bytes + co->single[c].bytes;
-    co->bytes += co->desc.bytes;
+    co->bytes += co->desc.bytes + co->pslot.bytes + co->todo.bytes;
     *out = co;
     return EPI_OK;
 }
@@ -460,6 +494,7 @@ CohortDev epi_cohort::host_desc() const
     }
     h.M = M;
     h.Mp = Mp;
+    h.pslot = (const uint32_t *)pslot.ptr;
     // class 1's planes as a byte offset from class 0's, when it fits 32 bits
     const size_t e1 = (size_t)Mp * 2 * EL, f1 = (size_t)Mp * 2 * FL;
     h.E1off = e1 < ((size_t)1 << 32) ? (uint32_t)e1 : 0;
@@ -478,6 +513,23 @@ extern "C" int epi_cohort_destroy(epi_cohort *co)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     delete co;
+    return EPI_OK;
+}
+
+extern "C" int epi_cohort_footprint(uint32_t n_samples, uint32_t n_snps, uint64_t *bytes)
+{
+    if (!bytes || !epi::cohort_shape_ok(n_samples, n_snps))
+        return EPI_EINVAL;
+    *bytes = epi::cohort_footprint(n_samples, n_snps, 3);
+    return EPI_OK;
+}
+
+extern "C" int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes)
+{
+    if (!ctx)
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->pair_budget = bytes ? bytes : ctx->pair_budget_default;
     return EPI_OK;
 }
 

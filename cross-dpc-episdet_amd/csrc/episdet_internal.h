@@ -11,6 +11,8 @@ constexpr int kLanes = 64;     // CDNA wavefront
 constexpr int kTop = 64;       // candidates kept per cohort
 constexpr int kWaveCap = 128;  // per-wavefront LDS candidate buffer
 constexpr int kMfmaWaves = 8;  // wavefronts per workgroup of the MFMA scan kernel (2 per SIMD)
+constexpr uint32_t kMaxSamples = 1u << 22;    // n_samples below this (exact FP4 sums)
+constexpr uint32_t kMaxSnps = (1u << 21) - 1;  // n_snps at most this (21-bit key fields)
 
 // Candidate: FP32 score and a lexicographic key (i << 42 | j << 21 | k).
 struct Cand {
@@ -35,7 +37,12 @@ constexpr uint64_t kEmptyKey = ~0ull;
 struct CohortDev {
     const uint4 *sm[2];   // SNP-major  [Mp][W[c]]   (uniform loads)
     const uint4 *wm[2];   // word-major [W[c]][Mp]   (per-lane loads)
-    uint4 *pair[2];       // [Mp][Mp], every (p, q): {n00, n01, n10, n11}
+    // pair margins {n00, n01, n10, n11} of (p, q), every p < Mp, in 64-column
+    // strips: column block q / 64 lives in strip slot pslot[q / 64] as
+    // [Mp rows][64], so (p, q) is at (pslot[q / 64] Mp + p) 64 + q % 64.
+    // A cohort holds every strip (pslot[g] = g) or, past its pair budget, a
+    // few slots that a scan refills per round (pair_at()).
+    uint4 *pair[2];
     uint2 *single[2];     // [Mp]: {count g0, count g1}
     const uint8_t *E[2];  // int8 MFMA variant: 0/1 bytes [Mp][2 planes][EL[c]], or null
     uint32_t EL[2];       // bytes per E row, both classes: 64 max(W) rounded up to 128, zero padded
@@ -45,8 +52,14 @@ struct CohortDev {
     uint32_t W[2];        // 64-bit words per class
     uint32_t n[2];        // class sizes
     uint32_t M, Mp;       // SNPs, padded to a multiple of 64
-    uint32_t pad[2];
+    const uint32_t *pslot;  // [Mp / 64] strip slot of each column block
 };
+
+// Offset of pair (p, q) in a class's strips, given the slot of q's block.
+__host__ __device__ inline size_t pair_at(uint32_t slot, uint32_t Mp, uint32_t p, uint32_t q)
+{
+    return ((size_t)slot * Mp + p) * 64 + (q & 63u);
+}
 
 // One (k-block, j-group) row of a cohort's triplet tiles.  Items of the
 // row are consecutive chunks of `waves` i values, i in [0, icount); item
@@ -122,8 +135,11 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
                        uint32_t W0, uint32_t W1, hipStream_t s);
 hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t *geno,
                              hipStream_t s);
-hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &host_cohort,
-                        hipStream_t s);
+// Pair-margin strips todo[t] = (column block, slot), t < n_todo (todo
+// null: strip t = block t in slot t).
+hipError_t launch_strips(const CohortDev *dev_cohort, const uint2 *todo, uint32_t n_todo,
+                         uint32_t Mp, hipStream_t s);
+hipError_t launch_singles(const CohortDev *dev_cohort, uint32_t Mp, hipStream_t s);
 bool scan_config_supported(const ScanConfig &cfg);
 // MFMA variants (episdet_mfma.hip); fp4 selects the FP4 kernel (variant 3)
 hipError_t launch_expand(const uint4 *sm, uint32_t Mp, uint32_t W, uint32_t EL, uint8_t *E,

@@ -8,7 +8,9 @@
 //   * pack_kernel    class split + bit-packing into 64-bit planes, two
 //                    layouts (SNP-major for scalar loads, word-major for
 //                    coalesced per-lane loads)          (GPU/v4.cpp:112-196)
-//   * pairs_kernel   pair and single margins of the {0,1} genotype planes
+//   * strips_kernel  pair margins of the {0,1} genotype planes, in 64-column
+//                    strips (all of them, or those one scan round needs)
+//   * singles_kernel single-SNP margins
 //   * scan_kernel    persistent; pulls (cohort, k-block, j-group, i-chunk)
 //                    items from a device queue.  A wavefront owns one i, a
 //                    group of snps_j consecutive j and 64 consecutive k (one
@@ -26,6 +28,8 @@
 //                    (~(g0|g1) under the padding mask) and scores it in FP32
 //                    and FP64.
 #include <math.h>
+
+#include <algorithm>
 
 #include "episdet_cells.h"
 #include "episdet_internal.h"
@@ -54,6 +58,10 @@ __device__ __forceinline__ uint2 ldc2(const uint2 *p)
 {
     const EPI_CONST uint32_t *q = (const EPI_CONST uint32_t *)p;
     return make_uint2(q[0], q[1]);
+}
+__device__ __forceinline__ uint32_t ldc1(const uint32_t *p)
+{
+    return *(const EPI_CONST uint32_t *)p;
 }
 
 __device__ __forceinline__ bool cand_less(float s, uint64_t k, float ts, uint64_t tk)
@@ -101,14 +109,18 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s)
 // packing (GPU/v4.cpp:112-196): order[] lists controls then cases, each in
 // original sample order; sample t of class c goes to bit t%64 of word t/64.
 // ------------------------------------------------------------------------
+// One-dimensional grid of (class, word, 256-SNP chunk) blocks, chunks
+// fastest (no grid dimension bounded by the word count).
 __global__ void __launch_bounds__(256)
 pack_kernel(const uint8_t *__restrict__ geno, uint32_t M, uint32_t Mp,
             const uint32_t *__restrict__ order, uint32_t n0, uint32_t n1,
-            uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1)
+            uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1,
+            uint32_t chunks, uint32_t wmax)
 {
-    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t w = blockIdx.y;
-    const uint32_t c = blockIdx.z;
+    const uint32_t b = blockIdx.x;
+    const uint32_t s = (b % chunks) * 256 + threadIdx.x;
+    const uint32_t w = (b / chunks) % wmax;
+    const uint32_t c = b / chunks / wmax;
     const uint32_t W = c ? W1 : W0;
     if (s >= Mp || w >= W)
         return;
@@ -139,9 +151,12 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
     const uint32_t wmax = W0 > W1 ? W0 : W1;
     if (wmax == 0)
         return hipSuccess;
-    dim3 grid((Mp + 255) / 256, wmax, 2);
-    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, geno, M, Mp, order, n0, n1,
-                       sm0, wm0, sm1, wm1, W0, W1);
+    const uint32_t chunks = (Mp + 255) / 256;
+    const uint64_t blocks = (uint64_t)chunks * wmax * 2;
+    if (blocks > 0x7FFFFFFFull)
+        return hipErrorInvalidConfiguration;  // callers bound the shape first
+    hipLaunchKernelGGL(pack_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, geno, M, Mp, order,
+                       n0, n1, sm0, wm0, sm1, wm1, W0, W1, chunks, wmax);
     return hipGetLastError();
 }
 
@@ -189,13 +204,26 @@ hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t
 // ------------------------------------------------------------------------
 // pair and single margins
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict__ cd)
+// Strips of pair margins: todo[t] = (column block g, slot) for t < n_todo
+// (todo null: strip t is block t in slot t).  One-dimensional grid of
+// (strip, class, 4-row group) blocks; wavefront r of a block computes row
+// p = 4 group + r, lane l column q = 64 g + l: X (row p's planes) is
+// wave-uniform, Z (column q's word-major planes) one coalesced 1 KiB load
+// per word and wavefront.
+__global__ void __launch_bounds__(256)
+strips_kernel(const CohortDev *__restrict__ cd, const uint2 *__restrict__ todo, uint32_t groups,
+              uint32_t t0)
 {
-    const uint32_t c = blockIdx.z;
-    const uint32_t p = blockIdx.y;
-    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t grp = b % groups;
+    const uint32_t c = (b / groups) & 1u;
+    const uint32_t t = t0 + b / groups / 2;
+    const uint32_t g = todo ? todo[t].x : t;
+    const uint32_t slot = todo ? todo[t].y : t;
     const uint32_t Mp = as_const(cd)->Mp;
-    if (q >= Mp)
+    const uint32_t p = grp * 4 + (threadIdx.x >> 6);
+    const uint32_t q = g * 64 + (threadIdx.x & 63u);
+    if (p >= Mp)
         return;
     const uint32_t W = as_const(cd)->W[c];
     const uint4 *X = as_const(cd)->sm[c] + (size_t)p * W;
@@ -209,18 +237,51 @@ __global__ void __launch_bounds__(256) pairs_kernel(const CohortDev *__restrict_
         a10 += __popc(x.z & z.x) + __popc(x.w & z.y);
         a11 += __popc(x.z & z.z) + __popc(x.w & z.w);
     }
-    // every (p, q), not only p < q: the MFMA kernel's epilogue scores whole
-    // 64 x 64 tiles, and with the full table the counts of its lanes outside
+    // every row p, not only p < q: the MFMA kernel's epilogue scores whole
+    // 64 x 64 tiles, and with every row the counts of its lanes outside
     // i < j < k are exact too (so every ln(n!) index stays in the table)
-    as_const(cd)->pair[c][(size_t)p * Mp + q] = make_uint4(a00, a01, a10, a11);
-    if (q == p)
-        as_const(cd)->single[c][p] = make_uint2(a00, a11);
+    as_const(cd)->pair[c][pair_at(slot, Mp, p, q)] = make_uint4(a00, a01, a10, a11);
 }
 
-hipError_t launch_pairs(const CohortDev *dev_cohort, const CohortDev &h, hipStream_t s)
+hipError_t launch_strips(const CohortDev *dev_cohort, const uint2 *todo, uint32_t n_todo,
+                         uint32_t Mp, hipStream_t s)
 {
-    dim3 grid((h.Mp + 255) / 256, h.Mp, 2);
-    hipLaunchKernelGGL(pairs_kernel, grid, dim3(256), 0, s, dev_cohort);
+    const uint32_t groups = (Mp + 3) / 4;
+    // at most 2^30 blocks per launch (2 x groups <= 2^20 blocks per strip)
+    const uint32_t per = (uint32_t)std::max<uint64_t>(1, (1ull << 30) / (2ull * groups));
+    for (uint32_t t0 = 0; t0 < n_todo; t0 += per) {
+        const uint32_t nt = std::min(per, n_todo - t0);
+        hipLaunchKernelGGL(strips_kernel, dim3(nt * 2 * groups), dim3(256), 0, s, dev_cohort, todo,
+                           groups, t0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+// Single-SNP margins {count g = 0, count g = 1} of every SNP and class.
+__global__ void __launch_bounds__(256) singles_kernel(const CohortDev *__restrict__ cd)
+{
+    const uint32_t Mp = as_const(cd)->Mp;
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 2 * Mp)
+        return;
+    const uint32_t c = t / Mp, p = t % Mp;
+    const uint32_t W = as_const(cd)->W[c];
+    const uint4 *X = as_const(cd)->sm[c] + (size_t)p * W;
+    uint32_t a0 = 0, a1 = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint4 x = X[w];
+        a0 += __popc(x.x) + __popc(x.y);
+        a1 += __popc(x.z) + __popc(x.w);
+    }
+    as_const(cd)->single[c][p] = make_uint2(a0, a1);
+}
+
+hipError_t launch_singles(const CohortDev *dev_cohort, uint32_t Mp, hipStream_t s)
+{
+    hipLaunchKernelGGL(singles_kernel, dim3((2 * Mp + 255) / 256), dim3(256), 0, s, dev_cohort);
     return hipGetLastError();
 }
 
@@ -385,6 +446,10 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
 
         const uint32_t n0 = C->n[0], n1 = C->n[1];
         const uint32_t tmax = P.table_len - 1;
+        // strip slots of the j group's and the k block's columns (the JP
+        // j share one 64-block)
+        const uint32_t slot_j = ldc1(C->pslot + (j0 >> 6));
+        const uint32_t slot_k = ldc1(C->pslot + kb);
 #pragma unroll
         for (int jj = 0; jj < JP; ++jj) {
             const uint32_t j = j0 + jj;
@@ -394,9 +459,9 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
                 uint32_t x0[27], x1[27];
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-                    const uint4 ij = ldc4(C->pair[c] + (size_t)i * Mp + j);
-                    const uint4 ik = C->pair[c][(size_t)i * Mp + k];
-                    const uint4 jk = C->pair[c][(size_t)j * Mp + k];
+                    const uint4 ij = ldc4(C->pair[c] + pair_at(slot_j, Mp, i, j));
+                    const uint4 ik = C->pair[c][pair_at(slot_k, Mp, i, k)];
+                    const uint4 jk = C->pair[c][pair_at(slot_k, Mp, j, k)];
                     const uint2 si = ldc2(C->single[c] + i);
                     const uint2 sj = ldc2(C->single[c] + j);
                     const uint2 sk = C->single[c][k];

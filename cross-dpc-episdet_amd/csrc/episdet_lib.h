@@ -113,9 +113,11 @@ bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps);
 uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant);
 // Packs a cohort into device memory on stream `s` (synchronized before
 // returning).  Caller holds ctx->mu.
+// full_pairs: every pair strip resident whatever the context's pair budget
+// (cohorts of the server, whose rounds read any strip).
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out, hipStream_t s);
+                         epi_cohort **out, hipStream_t s, bool full_pairs = false);
 void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<double> &t64);
 int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);
@@ -195,6 +197,10 @@ struct epi_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::mutex mu;
+    // device bytes of pair-margin strips a cohort keeps resident (both
+    // classes); a cohort whose full set would exceed it keeps a few strips
+    // that its scans refill (epi_set_pair_budget)
+    uint64_t pair_budget = 0, pair_budget_default = 0;
     // ln(n!) tables (FP32 reference semantics, FP64 twin), table_len entries
     uint32_t table_len = 0;
     epi::DevBuf t32, t64;
@@ -202,6 +208,7 @@ struct epi_ctx {
     // per-launch scratch, grown on demand
     epi::DevBuf round, fin, counters, slots, rescored;
     epi::DevBuf lists;  // running top-kTop lists of the jobs of one call
+    epi::DevBuf brows;  // row table of one round of a scan over strip slots
     // cohort creation inputs on the device (genotypes, sample order, .bed
     // bytes), kept so that creating a cohort frees nothing (a free waits for
     // the device, which may be running a scan of other cohorts)
@@ -226,6 +233,15 @@ struct epi_cohort {
     uint32_t n[2] = {0, 0};
     uint32_t W[2] = {0, 0};
     epi::DevBuf sm[2], wm[2], pair[2], single[2], desc;
+    // pair-margin strips: nslots slots of [Mp][64] per class; pslot (device,
+    // [Mp / 64]) maps a column block to its slot.  Not blocked: every block
+    // in its own slot.  Blocked: slot_block[t] is the block slot t holds
+    // (UINT32_MAX: none), set by the scan that fills it; todo is that scan's
+    // list of strips to compute.
+    uint32_t nslots = 0;
+    bool blocked = false;
+    std::vector<uint32_t> slot_block;
+    epi::DevBuf pslot, todo;
     // expanded planes of both classes, class 1 right after class 0, built on
     // first use: 0/1 bytes for the int8 MFMA variant (E), e2m1 nibbles for
     // the FP4 one (F); rows of one pitch (EL, FL bytes) in both classes

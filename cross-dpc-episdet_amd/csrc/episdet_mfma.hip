@@ -390,7 +390,8 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
     // SNP tid % 64 of the block
     const uint32_t c = (tid >> 7) & 1u, e = tid & 63u;
     const bool kb = (tid & 64u) != 0;
-    const uint4 x = C->pair[c][(size_t)i * Mp + (kb ? K0 : J0) + e];
+    const uint32_t q = (kb ? K0 : J0) + e;
+    const uint4 x = C->pair[c][pair_at(C->pslot[q >> 6], Mp, i, q)];
     const uint2 y = C->single[c][(kb ? K0 : J0) + e];
     if (kb) {
         G.ik[c][e] = x4(x);
@@ -687,12 +688,14 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             si[c] = make_uint2(s1.x << 2, s1.y << 2);
         }
         // (j, k) pair margins of the lane's slots, read a slot ahead: row
-        // J0 + j_local of pair[c], column k; every (p, q) < Mp is in the table
+        // J0 + j_local, column k of the k block's strip; every row p < Mp is
+        // in the strip
         const uint32_t jl0 = 4 * half + jq;  // slot 0's j_local
+        const uint32_t slot_k = mf::rfl(C->pslot[K0 >> 6]);
         const uint4 *jkp[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
-            jkp[c] = C->pair[c] + ((size_t)J0 + jl0) * Mp + k;
+            jkp[c] = C->pair[c] + pair_at(slot_k, Mp, J0 + jl0, k);
         uint4 jkn[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
@@ -703,9 +706,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 jkc[c] = x4(jkn[c]);
-                // slot s8 + 1, at its j_local offset from slot 0
+                // slot s8 + 1, at its j_local offset from slot 0 (rows of a
+                // strip are 64 entries apart)
                 if (s8 + 1 < 8)
-                    jkn[c] = jkp[c][(size_t)(32 * ((s8 + 1) >> 2) + 8 * ((s8 + 1) & 3)) * Mp];
+                    jkn[c] = jkp[c][(size_t)(32 * ((s8 + 1) >> 2) + 8 * ((s8 + 1) & 3)) * 64];
             }
             const uint32_t jl = jl0 + 32 * (s8 >> 2) + 8 * (s8 & 3);
             const uint32_t j = J0 + jl;

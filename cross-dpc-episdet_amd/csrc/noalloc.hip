@@ -85,7 +85,8 @@ using namespace epi;
 extern "C" int epi_scan_noalloc(epi_ctx *ctx, epi_cohort *co, const epi_tile *tile,
                                 epi_result *result)
 {
-    if (!ctx || !co || !result || co->ctx != ctx || (tile && tile->variant > 3))
+    // a cohort past its pair budget needs its strips recomputed per round
+    if (!ctx || !co || !result || co->ctx != ctx || (tile && tile->variant > 3) || co->blocked)
         return EPI_EINVAL;
     unsigned char *base = nullptr;
     {

@@ -504,6 +504,93 @@ void assemble_finished(const void *in, Job *const *jobs, uint32_t n, epi_result 
         status[b] = assemble(rs + (size_t)b * kTop, jobs[b]->co, results[b], 0.0, 0.0);
 }
 
+// Scan of a cohort that keeps only some pair strips (blocked): its column
+// blocks are cut into groups of s = nslots / 2; for each pair of groups
+// A <= B the strips of A go to slots [0, s) and those of B to slots
+// [s, 2 s) (none more when B = A), each computed only when its slot does not
+// hold it yet, the block -> slot map is uploaded, and one round scans the
+// rows whose j block is in A and whose k block is in B into the job's
+// running list.  Each row belongs to exactly one group pair, so every
+// triplet is scored once; every (j, k), (i, j) and (i, k) margin a round
+// reads is in the strip of the k block or of the j block.  kernel_ms (may
+// be null): summed device time of the rounds' scan kernels.
+static int scan_strip_rounds(epi_ctx *ctx, const LaunchPlan &plan, Job &job, float *dump_scores,
+                             uint32_t *dump_counts, double *kernel_ms)
+{
+    epi_cohort *co = job.co;
+    const uint32_t nb = co->Mp / 64;
+    const uint32_t s = std::max<uint32_t>(co->nslots / 2, 1);
+    const uint32_t G = (nb + s - 1) / s;
+    const uint32_t jp = (uint32_t)plan.cfg.snps_j;  // j per row group (divides 64)
+    std::vector<std::vector<Row>> bucket((size_t)G * G);
+    size_t most = 1;
+    for (const Row &r : job.rows) {
+        std::vector<Row> &b = bucket[(size_t)(r.jg * jp / 64 / s) * G + r.kb / s];
+        b.push_back(r);
+        most = std::max(most, b.size());
+    }
+    EPI_TRY(ctx->brows.reserve(sizeof(Row) * most));
+    // what the slots hold is known only for this scan's own uploads
+    std::fill(co->slot_block.begin(), co->slot_block.end(), 0xFFFFFFFFu);
+    std::vector<uint32_t> map(nb, 0);
+    std::vector<uint2> todo;
+    if (kernel_ms)
+        *kernel_ms = 0.0;
+    for (uint32_t ga = 0; ga < G; ++ga)
+        for (uint32_t gb = ga; gb < G; ++gb) {
+            std::vector<Row> &rows = bucket[(size_t)ga * G + gb];
+            if (rows.empty())
+                continue;
+            todo.clear();
+            auto want = [&](uint32_t g, uint32_t slot) {
+                if (co->slot_block[slot] != g) {
+                    co->slot_block[slot] = g;
+                    todo.push_back(make_uint2(g, slot));
+                }
+            };
+            for (uint32_t g = ga * s; g < std::min(nb, ga * s + s); ++g)
+                want(g, g - ga * s);
+            if (gb != ga)
+                for (uint32_t g = gb * s; g < std::min(nb, gb * s + s); ++g)
+                    want(g, s + g - gb * s);
+            std::fill(map.begin(), map.end(), 0u);
+            for (uint32_t t = 0; t < co->nslots; ++t)
+                if (co->slot_block[t] < nb)
+                    map[co->slot_block[t]] = t;
+            EPI_TRY(stage_upload(ctx, co->pslot.ptr, map.data(), sizeof(uint32_t) * nb));
+            if (!todo.empty()) {
+                EPI_TRY(stage_upload(ctx, co->todo.ptr, todo.data(), sizeof(uint2) * todo.size()));
+                EPI_HIP(launch_strips((const CohortDev *)co->desc.ptr, (const uint2 *)co->todo.ptr,
+                                      (uint32_t)todo.size(), co->Mp, ctx->stream));
+            }
+            // the group pair's rows, numbered from item 0
+            uint32_t item = 0;
+            for (Row &r : rows) {
+                r.item_start = item;
+                item += (r.icount + job.nw - 1) / job.nw;
+            }
+            EPI_TRY(stage_upload(ctx, ctx->brows.ptr, rows.data(), sizeof(Row) * rows.size()));
+            Job sub;
+            sub.co = co;
+            sub.rows = rows;
+            sub.items = item;
+            sub.nw = job.nw;
+            sub.list = job.list;
+            sub.drows = (const Row *)ctx->brows.ptr;
+            Job *sp[1] = {&sub};
+            const std::pair<size_t, size_t> all(0, rows.size());
+            EPI_TRY(enqueue_round(ctx, plan, sp, &all, 1, kernel_ms != nullptr, dump_scores,
+                                  dump_counts));
+            if (kernel_ms) {
+                float ms = 0.0f;
+                EPI_HIP(hipEventSynchronize(ctx->ev1));
+                EPI_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                *kernel_ms += ms;
+            }
+        }
+    return EPI_OK;
+}
+
 static int check_cohorts(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t n)
 {
     if (!ctx || !cohorts || n == 0)
@@ -537,6 +624,14 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
     EPI_TRY(reserve_lists(ctx, 1));
     EPI_TRY(job_init(ctx, job, cohort, cfg, (Cand *)ctx->lists.ptr));
     Job *jp[1] = {&job};
+    if (cohort->blocked) {
+        double kms = 0.0;
+        EPI_TRY(scan_strip_rounds(ctx, plan, job, nullptr, nullptr, &kms));
+        EPI_TRY(enqueue_rescore(ctx, jp, 1));
+        EPI_TRY(copy_back_and_assemble(ctx, jp, 1, result, t0, false));
+        result->kernel_ms = kms;
+        return EPI_OK;
+    }
     const std::pair<size_t, size_t> all(0, job.rows.size());
     EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, true, nullptr, nullptr));
     EPI_TRY(enqueue_rescore(ctx, jp, 1));
@@ -558,8 +653,11 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     std::lock_guard<std::mutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     uint32_t n_max = 0;
-    for (uint32_t b = 0; b < n; ++b)
+    for (uint32_t b = 0; b < n; ++b) {
+        if (cohorts[b]->blocked)
+            return EPI_EINVAL;  // one launch reads every strip: scan it alone
         n_max = std::max(n_max, cohorts[b]->N);
+    }
     EPI_TRY(ctx_ensure_tables(ctx, n_max));
     const ScanConfig cfg = resolve_config(ctx, tile, n_max);
     LaunchPlan plan;
@@ -648,7 +746,11 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
     EPI_TRY(job_init(ctx, job, cohort, cfg, nullptr));
     Job *jp[1] = {&job};
     const std::pair<size_t, size_t> all(0, job.rows.size());
-    EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, false, (float *)ds.ptr, (uint32_t *)dc.ptr));
+    if (cohort->blocked)
+        EPI_TRY(scan_strip_rounds(ctx, plan, job, (float *)ds.ptr, (uint32_t *)dc.ptr, nullptr));
+    else
+        EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, false, (float *)ds.ptr,
+                              (uint32_t *)dc.ptr));
     EPI_HIP(hipMemcpyAsync(scores, ds.ptr, ntrip * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream));
     if (counts)

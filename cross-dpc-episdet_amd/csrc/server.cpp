@@ -151,7 +151,7 @@ int epi_server::admit_locked_ctx(Request *r)
     int st = EPI_OK;
     if (!r->resident)
         st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
-                                  &r->co, upload);
+                                  &r->co, upload, true);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg, nullptr);
     if (st == EPI_OK && r->resident) {
@@ -579,7 +579,9 @@ extern "C" int epi_submit(epi_server *s, const uint8_t *geno, const uint8_t *phe
 
 extern "C" int epi_submit_cohort(epi_server *s, epi_cohort *co, uint64_t *request_id)
 {
-    if (!s || !co || !request_id || co->ctx != s->ctx || co->N > s->cfg.max_samples)
+    // a cohort past its pair budget is scanned in strip rounds by epi_scan only
+    if (!s || !co || !request_id || co->ctx != s->ctx || co->N > s->cfg.max_samples ||
+        co->blocked)
         return EPI_EINVAL;
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r)

@@ -100,6 +100,8 @@ _PROTOS = [
     ("epi_cohort_create", ctypes.c_int, [_VP, _VP, _VP, _u32, _u32, ctypes.POINTER(_VP)]),
     ("epi_cohort_destroy", ctypes.c_int, [_VP]),
     ("epi_cohort_bytes", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    ("epi_cohort_footprint", ctypes.c_int, [_u32, _u32, ctypes.POINTER(ctypes.c_uint64)]),
+    ("epi_set_pair_budget", ctypes.c_int, [_VP, ctypes.c_uint64]),
     ("epi_scan", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), ctypes.POINTER(EpiResult)]),
     ("epi_scan_batch", ctypes.c_int, [_VP, ctypes.POINTER(_VP), _u32, ctypes.POINTER(EpiTile),
                                       ctypes.POINTER(EpiResult)]),
@@ -221,6 +223,14 @@ def read_raw(geno_path: str, pheno_path: str, n_samples: int):
 
 
 # ---------------------------------------------------------------- generators
+def cohort_footprint(n_samples: int, n_snps: int) -> int:
+    """Device bytes of a cohort of this shape (epi_cohort_footprint); raises
+    EpiError(EPI_EINVAL) for a shape no cohort may have.  Needs no GPU."""
+    b = ctypes.c_uint64()
+    _check(lib().epi_cohort_footprint(n_samples, n_snps, ctypes.byref(b)), "epi_cohort_footprint")
+    return int(b.value)
+
+
 def generate_reference(n_samples: int, n_snps: int):
     """The reference's srand(100) generator (GPU/v4.cpp:81-110)."""
     g = np.empty((n_samples, n_snps), np.uint8)
@@ -346,6 +356,11 @@ class Context:
         _check(lib().epi_scan(self.handle, cohort.handle, ctypes.byref(t) if t else None,
                               ctypes.byref(r)), "epi_scan")
         return ScanResult.from_c(r)
+
+    def set_pair_budget(self, nbytes: int) -> None:
+        """Pair-margin bytes cohorts created afterwards keep resident (0:
+        the default); past it a cohort is scanned in strip rounds."""
+        _check(lib().epi_set_pair_budget(self.handle, int(nbytes)), "epi_set_pair_budget")
 
     def scan_noalloc(self, cohort: "Cohort", tile=None) -> ScanResult:
         """epi_scan_noalloc: the same scan with no device allocation (a

@@ -118,6 +118,22 @@ int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
 int epi_cohort_destroy(epi_cohort *cohort);
 /* Device bytes held by a cohort (its admission footprint). */
 int epi_cohort_bytes(const epi_cohort *cohort, uint64_t *bytes);
+/* Device bytes a cohort of this shape holds with every pair strip resident
+ * and FP4 expanded planes (the server's admission unit), or EPI_EINVAL for
+ * a shape no cohort may have: n_samples in [1, 2^22) (the FP4 matrix sums
+ * stay exact), n_snps in [3, 2^21 - 1] (21-bit indices), n_samples x
+ * n_snps <= 2^38.  Needs no device.  No reference counterpart. */
+int epi_cohort_footprint(uint32_t n_samples, uint32_t n_snps, uint64_t *bytes);
+/* Most device bytes of pair margins (both classes) a cohort created after
+ * this call keeps resident; 0 restores the default, an eighth of device
+ * memory.  A cohort whose pair margins exceed it (32 x padded n_snps^2
+ * bytes) keeps room for budget / (2048 x padded n_snps) 64-column strips,
+ * at least two, and epi_scan / epi_scan_dump then scan it in rounds that
+ * recompute the strips each round needs; such a cohort is refused
+ * (EPI_EINVAL) by epi_scan_batch, epi_scan_noalloc and the server.  The
+ * reference keeps no pair margins (its memory is O(n_snps x n_samples),
+ * GPU/v4.cpp:256-257); no counterpart. */
+int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes);
 
 /* Scans ---------------------------------------------------------------- */
 /* Exhaustive scan of all C(M,3) triplets.  Replaces the block-triple

@@ -110,3 +110,19 @@ def test_strerror_names_every_code(epi):
     for code in range(0, -11, -1):
         s = epi.strerror(code)
         assert s and "unknown" not in s
+
+
+def test_cohort_shape_bounds(epi):
+    """epi_cohort_footprint accepts exactly the shapes every kernel can
+    launch and count exactly (no device needed): n_samples in [1, 2^22),
+    n_snps in [3, 2^21 - 1], n_samples x n_snps <= 2^38."""
+    ok = [(1, 3), (1024, 256), ((1 << 22) - 1, 3), (1, (1 << 21) - 1), (1 << 17, 1 << 20)]
+    bad = [(0, 10), (10, 2), (10, 0), (1 << 22, 3), (1, 1 << 21), (1 << 21, (1 << 17) + 1)]
+    for N, M in ok:
+        b = epi.cohort_footprint(N, M)
+        Mp = (M + 63) // 64 * 64
+        assert b >= 32 * Mp * Mp  # the full pair margins of both classes
+    for N, M in bad:
+        with pytest.raises(epi.EpiError) as e:
+            epi.cohort_footprint(N, M)
+        assert e.value.code == epi.EPI_EINVAL, (N, M)
