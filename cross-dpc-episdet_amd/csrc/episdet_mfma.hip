@@ -12,10 +12,10 @@
 //   FP4  (variant 3): e2m1 nibbles 0 and 1.0 (0x2), two samples per byte,
 //        v_mfma_scale_f32_32x32x64_f8f6f4 (block scale 2^0 on A, 2^2 on B),
 //        FP32 accumulation.  Every product is 0 or 4 and every partial sum an
-//        integer below 2^24 (class sizes are below 2^21), so the FP32 sums are
-//        exact (4 x the counts); the instruction does twice the int8
-//        instruction's MACs in the same issue time, and the operands take
-//        half the bytes.
+//        integer below 2^24 (class sizes are below 2^22, which cohort
+//        creation enforces), so the FP32 sums are exact (4 x the counts);
+//        the instruction does twice the int8 instruction's MACs in the same
+//        issue time, and the operands take half the bytes.
 // The operand bytes come from a per-cohort expansion of the bit planes
 // (E[c] or F[c], [snp][plane][sample]).  Both formats use one LDS layout:
 // 128-byte rows per stage, 32 bytes per MFMA step (32 samples as bytes, 64
@@ -23,19 +23,18 @@
 // over the K dimension covers each sample once whatever that mapping is.
 //
 // Work item = (i, j-block of 64 SNPs, k-block of 64 SNPs), one workgroup of
-// 4 wavefronts, one per SIMD.  Per stage the workgroup copies the j-block
-// and k-block rows (and SNP i's) into LDS; wavefront w computes the 64 rows
-// (16 j x 4 (a,b); its j are the block's SNPs w, w + 4, ..., w + 60, so that
-// the pairs outside i < j < k spread evenly over the wavefronts) against all
-// 128 columns (64 k x 2 c): 2 x 4 MFMA tiles of 32 x 32 per class, 256
-// accumulators in AGPRs.
-// Pipelining: two LDS stage buffers, one barrier per stage.  Steps 0 and 1
-// of a stage run before its barrier, steps 2 and 3 after it (operands in
-// registers), and the next stage's copies (global_load_lds, no registers)
-// are issued between those MFMAs; each step's operand reads are issued a
-// step ahead.  The epilogue rebuilds the 27-cell tables exactly from the 8
-// counted cells and the pair and single margins, sums K2 in the reference
-// order and keeps a per-wavefront top-64.
+// 8 wavefronts, two per SIMD.  Per stage the workgroup copies the j-block
+// and k-block rows (and SNP i's) into LDS; wavefront w (jq = w % 4,
+// kq = w / 4) computes 64 rows (16 j x 4 (a,b); its j are the block's SNPs
+// jq, jq + 4, ..., jq + 60, so that the pairs outside i < j < k spread
+// evenly over the wavefronts) against 64 columns (32 k of half kq, x 2 c):
+// 2 x 2 MFMA tiles of 32 x 32 per class, 128 accumulators for both classes.
+// Pipelining: two LDS stage buffers, one barrier per stage; the next
+// stage's copies (global_load_lds, no registers) are issued at the start of
+// a stage, and each step's operand reads a step ahead.  The epilogue
+// rebuilds the 27-cell tables exactly from the 8 counted cells and the pair
+// and single margins, sums K2 in the reference order and keeps a
+// per-wavefront top-64.
 #include <math.h>
 
 #include "episdet_cells.h"
@@ -211,6 +210,7 @@ constexpr int kRowBytes = kStage;     // LDS bytes per staged (snp, plane) row
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
 constexpr int kIChunks = 2 * kStage / 16 / kMfWaves;  // 16-byte chunks of SNP i's rows per wave
+constexpr int kGather = 9;  // K2 cells whose table reads are issued ahead of their sums (divides 27)
 static_assert(kStageCopies == 4 && kIChunks == 2, "copy split");
 
 // Staged rows are 128 bytes with no padding; 16-byte slot s of row r holds
@@ -704,12 +704,19 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         for (uint32_t s8 = 0; s8 < 8; ++s8) {
             uint4 jkc[2];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
+            for (int c = 0; c < 2; ++c)
                 jkc[c] = x4(jkn[c]);
-                // slot s8 + 1, at its j_local offset from slot 0 (rows of a
-                // strip are 64 entries apart)
-                if (s8 + 1 < 8)
-                    jkn[c] = jkp[c][(size_t)(32 * ((s8 + 1) >> 2) + 8 * ((s8 + 1) & 3)) * 64];
+            // slot s8 + 1, at its j_local offset from slot 0 (rows of a strip
+            // are 64 entries apart).  Both loads are unconditional (the last
+            // slot reloads its own row) and follow both copies above, so the
+            // wait for them sits at the top of the next slot, a slot's work
+            // after their issue, instead of on this slot's path.
+            {
+                const uint32_t nx = s8 + 1 < 8 ? s8 + 1 : s8;
+                const size_t off = (size_t)(32 * (nx >> 2) + 8 * (nx & 3)) * 64;
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    jkn[c] = jkp[c][off];
             }
             const uint32_t jl = jl0 + 32 * (s8 >> 2) + 8 * (s8 & 3);
             const uint32_t j = J0 + jl;
@@ -767,13 +774,13 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
                 // summed over the cells in order from 0 (the first term
                 // needs no add: no term is -0, so 0 + t == t bit for bit);
-                // the table reads of 9 cells are issued ahead of their
+                // the table reads of kGather cells are issued ahead of their
                 // ordered sums.
 #pragma unroll
-                for (int m0 = 0; m0 < 27; m0 += 9) {
-                    float ta[9], tb[9], td[9];
+                for (int m0 = 0; m0 < 27; m0 += kGather) {
+                    float ta[kGather], tb[kGather], td[kGather];
 #pragma unroll
-                    for (int m = 0; m < 9; ++m) {
+                    for (int m = 0; m < kGather; ++m) {
                         const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
                         ta[m] = tab_at(tab + 4, ob + od);
                         tb[m] = tab_at(tab, ob);
@@ -781,7 +788,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                     }
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int m = 0; m < 9; ++m) {
+                    for (int m = 0; m < kGather; ++m) {
                         const float term = (ta[m] - tb[m]) - td[m];
                         sc = (m0 + m == 0) ? term : sc + term;
                     }

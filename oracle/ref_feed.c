@@ -10,7 +10,7 @@
  *   - genotype draws: rand() % 3 == value, so a stored 0/1/2 is returned as is;
  *   - phenotype draws: a value r with (int)(N * ((double)r / RAND_MAX)) equal
  *     to the chosen case index.
- * The stream is prepared by tests/golden/make_ref_vectors.py.
+ * The stream is prepared by tests/golden/make_golden.py.
  */
 #include <stdio.h>
 #include <stdlib.h>
