@@ -59,7 +59,19 @@ def reference_best(spec, exe, td):
         raise RuntimeError(f"unexpected output {out!r}")
     return {"best": [int(m.group(1)), int(m.group(2)), int(m.group(3))], "score": m.group(4),
             "planted": list(planted) if planted else None,
-            "program": f"{os.path.basename(exe)} {block_pac} 16", "seconds": time.time() - t0}
+            "program": f"{os.path.basename(exe)} {block_pac} 16", "seconds": time.time() - t0,
+            "host": host_name()}
+
+
+def host_name():
+    """CPU model and logical CPUs of the host that ran the program."""
+    model = "unknown CPU"
+    with open("/proc/cpuinfo") as f:
+        for line in f:
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    return f"{model}, {len(os.sched_getaffinity(0))} logical CPUs"
 
 
 def main():
