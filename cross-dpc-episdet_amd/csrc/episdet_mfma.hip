@@ -296,7 +296,10 @@ __device__ __forceinline__ v4i lds16a(uint32_t addr)
 // counts these copies as LDS operations too and then waits for every
 // outstanding LDS read, keeps exact counts for the operand reads;
 // completion is awaited explicitly (s_waitcnt vmcnt) before the barrier
-// that publishes a stage.  The instruction has no register result.
+// that publishes a stage.  The instruction has no register result.  M0 is
+// a reserved register the compiler may not preserve across the asm; no
+// compiler-generated instruction of these kernels reads or writes it
+// (tests/test_isa.py checks the gfx950 assembly).
 __device__ __forceinline__ void lds_copy16(const uint8_t *base, uint32_t off, uint32_t lds)
 {
     asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1"
