@@ -228,8 +228,9 @@ struct MfStage {
 static_assert(__builtin_offsetof(MfStage, krows) == kStageRows * kRowBytes, "k rows follow j rows");
 
 // The margins of SNP i and the single-SNP counts an item's epilogue reads,
-// staged in LDS over the (then idle) stage buffers, x 4.  The (j, k) pair
-// margins are read from global memory by each lane, a slot ahead.
+// staged in LDS x 4, in an area of their own: the stage buffers stay free
+// during the epilogue and receive the next item's first stage.  The (j, k)
+// pair margins are read from global memory by each lane, a slot ahead.
 struct MfMargins {
     uint4 ij[2][64];            // pair margins of (i, j)
     uint4 ik[2][64];            // pair margins of (i, k)
@@ -240,10 +241,8 @@ struct MfMargins {
 struct MfLds {
     Cand cand[kMfWaves][kWaveCap + kTop];
     uint32_t misc[4];
-    union {
-        MfStage stage[2];  // double buffer: one is read while the other is filled
-        MfMargins mg;      // the epilogue's margins (after the stage loop)
-    };
+    MfMargins mg;      // the epilogue's margins
+    MfStage stage[2];  // double buffer: one is read while the other is filled
 };
 
 // The kernel's LDS is one static block.  With the ln(n!) table in LDS the
@@ -490,12 +489,16 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     const uint32_t eoff = (uint32_t)__builtin_offsetof(MfStage, irow) + a_lane * kStage + 16 * half;
 
     // Work-queue items: the next item is claimed while this one runs.
+    // Thread 0 publishes the claim in misc[0] after the item's stage loop
+    // (every wavefront read misc[0] before the stage loop's barriers) and
+    // the margin barrier makes it visible, so the epilogue knows the next
+    // item and copies its first stage into the idle stage buffer 0 (`pre`).
     if (tid == 0)
         L.misc[0] = atomicAdd(&P.counters[0], 1u);
     __syncthreads();
     uint32_t item = mf::rfl(L.misc[0]);
+    bool pre = false;
     for (;;) {
-        __syncthreads();  // every wavefront has read misc[0]
         uint32_t claimed = 0;
         if (tid == 0 && item < n_items)
             claimed = atomicAdd(&P.counters[0], 1u);
@@ -524,10 +527,8 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         }
         if (item >= n_items)
             break;
-        if (!DUMP) {
-            Wt.gthr = fminf(Wt.gthr, topk::load_thr(P.thr, cohort));
+        if (!DUMP)
             Wt.thr_slot = P.thr + cohort;
-        }
 
         const uint32_t K0 = rows[r].kb * 64;
         const uint32_t J0 = rows[r].jg * 64;
@@ -571,11 +572,14 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             const MfCopyOffsets co = mf_copy_offsets(Lc, wave, lane);
             const uint8_t *blk_base = E + (size_t)((wave >> 2) ? K0 : J0) * 2 * Lc;
             const uint8_t *i_base = E + (size_t)i * 2 * Lc;
-            if (!(c == 1 && staged)) {
+            if (pre) {
+                // the previous item's epilogue copied this stage into buffer 0
+                pre = false;
+                par = 0;
+            } else if (!(c == 1 && staged)) {
                 // Both buffers are free here: the previous stage loop ended
-                // with a barrier, and the previous item's epilogue (which
-                // reuses them for margins) ended with the barrier before the
-                // next claim.
+                // with a barrier, and the stage buffers are not read during
+                // an epilogue.
                 par = 0;
 #pragma unroll
                 for (int q = 0; q < kStageCopies; ++q)
@@ -674,11 +678,50 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
         // not unrolled (one copy of the scoring code); a switch reads the
         // slot's 16 accumulators.
         const uint32_t n04 = C->n[0] << 2, n14 = C->n[1] << 2;
-        // the stage buffers are no longer read
-        __syncthreads();
+        // The previous item's margins were read before this item's first
+        // stage barrier, so they are overwritten freely.
+        if (tid == 0)
+            L.misc[0] = claimed;
         if (tid < 256)
             mf_stage_margins(L.mg, C, i, J0, K0, Mp, tid);
         __syncthreads();
+        const uint32_t next = mf::rfl(L.misc[0]);
+        if (!DUMP)
+            Wt.gthr = fminf(Wt.gthr, topk::load_thr(P.thr, cohort));
+        // The next item's first stage (of its first class with samples)
+        // goes to stage buffer 0 now: no stage buffer is read until the
+        // next item's first stage barrier, and these copies land while the
+        // slot loop below waits for its first pair margins.  The walk to
+        // the next item works on copies of the walk state.
+        if (next < n_items) {
+            uint32_t sg2 = sg, r2 = r;
+            bool fresh2 = false;
+            while (sg2 + 1 < n_segs && segs[sg2 + 1].item_base <= next) {
+                ++sg2;
+                fresh2 = true;
+            }
+            const EPI_CONST Row *rows2 = (const EPI_CONST Row *)segs[sg2].rows;
+            if (fresh2)
+                r2 = segs[sg2].row_begin;
+            const uint32_t u2 = next - segs[sg2].item_base + segs[sg2].item_first;
+            const uint32_t row_end2 = segs[sg2].row_end;
+            while (r2 + 1 < row_end2 && rows2[r2 + 1].item_start <= u2)
+                ++r2;
+            const EPI_CONST CohortDev *C2 = mf::as_const(P.cohorts) + segs[sg2].cohort;
+            const uint32_t c2 = C2->W[0] > 0 ? 0u : 1u;
+            if (C2->W[c2] > 0) {
+                const uint8_t *E2 = FP4 ? C2->F[c2] : C2->E[c2];
+                const uint32_t Lc2 = FP4 ? C2->FL[c2] : C2->EL[c2];
+                const MfCopyOffsets co2 = mf_copy_offsets(Lc2, wave, lane);
+                const uint32_t b2 = ((wave >> 2) ? rows2[r2].kb : rows2[r2].jg) * 64;
+                const uint32_t i2 = u2 - rows2[r2].item_start;
+#pragma unroll
+                for (int q = 0; q < kStageCopies; ++q)
+                    mf_issue_copy(stage_lds[0], E2 + (size_t)b2 * 2 * Lc2, co2, wave, q);
+                mf_issue_icopy(stage_lds[0], E2 + (size_t)i2 * 2 * Lc2, co2, wave, lane);
+                pre = true;
+            }
+        }
         const uint32_t kl = 32 * kq + l32;
         const uint32_t k = K0 + kl;
         uint4 ik[2];
@@ -803,10 +846,10 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             if (!DUMP)
                 topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
         }
-        if (tid == 0)
-            L.misc[0] = claimed;
+        // The wavefronts leave the epilogue together (measured faster than
+        // letting the early ones run ahead into the next stage loop).
         __syncthreads();
-        item = mf::rfl(L.misc[0]);
+        item = next;
     }
 }
 // Static LDS of the instance; a table too large for LDS reports more than
