@@ -62,26 +62,56 @@ struct Compacted {
     uint64_t tk;
 };
 
-// Kept out of line (it runs rarely); state goes in and out by value so the
-// caller's registers are not forced into scratch memory.
-__device__ __noinline__ Compacted compact_buf(Cand *buf, Cand *keep, int n, float ts, uint64_t tk,
-                                              uint32_t lane)
+typedef __attribute__((address_space(3))) Cand LdsCand;
+
+__device__ __forceinline__ Cand ld_cand(const LdsCand *p)
 {
+    Cand c;
+    c.s = p->s;
+    c.pad = p->pad;
+    c.key = p->key;
+    return c;
+}
+__device__ __forceinline__ void st_cand(LdsCand *p, const Cand &c)
+{
+    p->s = c.s;
+    p->pad = c.pad;
+    p->key = c.key;
+}
+
+// Kept out of line (it runs rarely); state goes in and out by value so the
+// caller's registers are not forced into scratch memory.  The buffers are
+// passed as LDS pointers: a generic pointer would make every access a FLAT
+// instruction waited on with both counters.  Each lane ranks its entries
+// against all n, reading 8 (broadcast) entries per wait.
+__device__ __noinline__ Compacted compact_buf(LdsCand *buf, LdsCand *keep, int n, float ts,
+                                              uint64_t tk, uint32_t lane)
+{
+    constexpr int kRankBatch = 8;
     wave_sync();
     for (int e = (int)lane; e < n; e += kLanes) {
-        const Cand me = buf[e];
-        int rank = 0;
-        for (int f = 0; f < n; ++f) {
-            const Cand o = buf[f];
+        const Cand me = ld_cand(buf + e);
+        int rank = 0, f = 0;
+        for (; f + kRankBatch <= n; f += kRankBatch) {
+            Cand o[kRankBatch];
+#pragma unroll
+            for (int q = 0; q < kRankBatch; ++q)
+                o[q] = ld_cand(buf + f + q);
+#pragma unroll
+            for (int q = 0; q < kRankBatch; ++q)
+                rank += less(o[q].s, o[q].key, me.s, me.key) ? 1 : 0;
+        }
+        for (; f < n; ++f) {
+            const Cand o = ld_cand(buf + f);
             rank += less(o.s, o.key, me.s, me.key) ? 1 : 0;
         }
         if (rank < kTop)
-            keep[rank] = me;
+            st_cand(keep + rank, me);
     }
     wave_sync();
     const int m = n < kTop ? n : kTop;
     if ((int)lane < m)
-        buf[lane] = keep[lane];
+        st_cand(buf + lane, ld_cand(keep + lane));
     Compacted r;
     r.cnt = m;
     r.ts = ts;
@@ -99,7 +129,7 @@ __device__ __noinline__ Compacted compact_buf(Cand *buf, Cand *keep, int n, floa
 // filter early.
 __device__ __forceinline__ void wave_compact(WaveTop &W, uint32_t lane)
 {
-    const Compacted r = compact_buf(W.buf, W.keep, W.cnt, W.ts, W.tk, lane);
+    const Compacted r = compact_buf((LdsCand *)W.buf, (LdsCand *)W.keep, W.cnt, W.ts, W.tk, lane);
     W.cnt = __builtin_amdgcn_readfirstlane(r.cnt);
     W.ts = r.ts;
     W.tk = r.tk;
