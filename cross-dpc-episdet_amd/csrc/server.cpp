@@ -195,28 +195,41 @@ void epi_server::release_locked_ctx(Request *r)
 
 // Launches one scheduling round over `act` (ctx->mu held), followed by the
 // rescoring of the requests it finishes; `units` is its work estimate.
+// Fraction of a round's budget reserved for the oldest active request.
+constexpr double kOldestShare = 0.9;
+
 int epi_server::round_launch(std::vector<Request *> &act, double &units)
 {
     const uint32_t cap = cfg.max_active;
     const uint32_t n = (uint32_t)act.size();
-    // Max-min fair shares of the round budget: a request that needs less
-    // than an equal share gets what it needs, and what it leaves is split
-    // among the others, so rounds stay full while small requests finish in
-    // their first round.  At least one row each.
+    // Shares of the round budget.  The oldest request (act is in admission
+    // order) first gets up to kOldestShare of it: a large request then runs
+    // at no less than that fraction of the speed it would have alone, which
+    // bounds the tail latency against a one-at-a-time FIFO queue.  The rest
+    // is split max-min fair among all of them (the oldest for what it still
+    // needs): a request that needs less than an equal share gets what it
+    // needs and what it leaves goes to the others, so rounds stay full while
+    // small requests finish in their first round.  At least one row each.
     const double budget = std::max(units_per_ms, 1.0) * cfg.round_ms;
     std::vector<double> need(n, 0.0), share(n, 0.0);
     for (uint32_t b = 0; b < n; ++b)
         need[b] = act[b]->cum.back() - act[b]->cum[act[b]->job.next_row];
+    double left = budget;
+    if (n > 0) {
+        share[0] = std::min(need[0], kOldestShare * budget);
+        need[0] -= share[0];
+        left -= share[0];
+    }
     std::vector<uint32_t> by_need(n);
     for (uint32_t b = 0; b < n; ++b)
         by_need[b] = b;
     std::sort(by_need.begin(), by_need.end(),
               [&](uint32_t a, uint32_t c) { return need[a] < need[c]; });
-    double left = budget;
     for (uint32_t q = 0; q < n; ++q) {
         const uint32_t b = by_need[q];
-        share[b] = std::min(need[b], left / (double)(n - q));
-        left -= share[b];
+        const double s = std::min(need[b], left / (double)(n - q));
+        share[b] += s;
+        left -= s;
     }
     std::vector<Job *> jp(n);
     std::vector<std::pair<size_t, size_t>> ranges(n);
