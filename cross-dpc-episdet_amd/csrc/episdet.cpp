@@ -283,6 +283,7 @@ epi_cohort::~epi_cohort()
     }
     E.release();
     F.release();
+    perm.release();
     desc.release();
 }
 
@@ -291,7 +292,7 @@ void epi_cohort::use_pool(epi::DevPool *p)
     for (int c = 0; c < 2; ++c) {
         sm[c].pool = wm[c].pool = pair[c].pool = single[c].pool = p;
     }
-    E.pool = F.pool = desc.pool = drows.pool = pslot.pool = todo.pool = p;
+    E.pool = F.pool = perm.pool = desc.pool = drows.pool = pslot.pool = todo.pool = p;
 }
 
 namespace epi {
@@ -308,7 +309,8 @@ bool cohort_shape_ok(uint32_t n_samples, uint32_t n_snps)
 }
 
 // Device bytes a cohort of this shape may hold when scanned by `variant`
-// (planes in two layouts, pair and single margins, descriptor, and the
+// (planes in two layouts, pair and single margins, the stored SNP order,
+// descriptor, and the
 // expansion an MFMA scan adds: 2 classes x 2 planes x Mp rows of one pitch,
 // the larger class's samples rounded up to a 128-byte stage, at most
 // n_samples / 2 + 128 bytes as nibbles and n_samples + 128 as bytes), the
@@ -321,8 +323,8 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant)
     const uint64_t expanded = variant == 3 ? 4 * Mp * (N / 2 + 128)
                             : variant == 2 ? 4 * Mp * (N + 128)
                                            : 0;
-    return 2 * words * Mp * 16 + 2 * Mp * Mp * 16 + 2 * Mp * 8 + expanded + sizeof(CohortDev) +
-           4096;
+    return 2 * words * Mp * 16 + 2 * Mp * Mp * 16 + 2 * Mp * 8 + 2 * Mp * 4 + expanded +
+           sizeof(CohortDev) + 4096;
 }
 
 // Caller holds ctx->mu.  Exactly one of geno (sample-major bytes) and bed
@@ -416,12 +418,15 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     if (st == EPI_OK && e == hipSuccess)
         e = hipMemcpyAsync(dorder.ptr, order.data(), (size_t)kept * 4, hipMemcpyHostToDevice,
                            strm);
+    auto pack = [&](const uint32_t *src) {
+        return launch_pack((const uint8_t *)dgeno.ptr, n_samples, n_snps, co->Mp, src,
+                           (const uint32_t *)dorder.ptr, co->n[0], co->n[1],
+                           (uint4 *)co->sm[0].ptr, (uint4 *)co->wm[0].ptr,
+                           (uint4 *)co->sm[1].ptr, (uint4 *)co->wm[1].ptr, co->W[0], co->W[1],
+                           strm);
+    };
     if (st == EPI_OK && e == hipSuccess)
-        e = launch_pack((const uint8_t *)dgeno.ptr, n_samples, n_snps, co->Mp,
-                        (const uint32_t *)dorder.ptr, co->n[0], co->n[1],
-                        (uint4 *)co->sm[0].ptr, (uint4 *)co->wm[0].ptr,
-                        (uint4 *)co->sm[1].ptr, (uint4 *)co->wm[1].ptr, co->W[0], co->W[1],
-                        strm);
+        e = pack(nullptr);
     CohortDev h = co->host_desc();
     if (st == EPI_OK && e == hipSuccess)
         e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm);
@@ -432,6 +437,50 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     // of the resident cohort: every scan reads them, none recomputes them
     if (st == EPI_OK && e == hipSuccess)
         e = launch_singles((const CohortDev *)co->desc.ptr, co->Mp, strm);
+    // Stored SNP order: by allele frequency (count of allele 0 over both
+    // classes, 2 n(g = 0) + n(g = 1); ties by original index), so that the
+    // 64 SNPs of a k-block have similar genotype counts.  The planes are
+    // packed again in that order; results map back to original indices.
+    // Lanes that may reach the top list and whose SNPs are out of original
+    // order are summed twice, which costs more than the order saves until
+    // each wavefront's threshold has settled: below kOrderMinSnps SNPs the
+    // scan is too short for that (by default the order is used from there).
+    std::vector<uint32_t> perm;
+    const bool ordered = ctx->snp_order == 1 || (ctx->snp_order == 2 && n_snps >= kOrderMinSnps);
+    if (st == EPI_OK && e == hipSuccess && ordered)
+        st = co->perm.reserve((size_t)2 * co->Mp * sizeof(uint32_t));
+    if (st == EPI_OK && e == hipSuccess && ordered) {
+        std::vector<uint2> sg((size_t)2 * co->Mp);
+        for (int c = 0; c < 2 && e == hipSuccess; ++c)
+            e = hipMemcpyAsync(sg.data() + (size_t)c * co->Mp, co->single[c].ptr,
+                               (size_t)co->Mp * sizeof(uint2), hipMemcpyDeviceToHost, strm);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(strm);
+        if (e == hipSuccess) {
+            const uint32_t Mp = co->Mp;
+            std::vector<uint64_t> key(n_snps);
+            for (uint32_t p = 0; p < n_snps; ++p) {
+                const uint2 a = sg[p], b = sg[Mp + p];
+                key[p] = (uint64_t)(2 * (a.x + b.x) + (a.y + b.y)) << 32 | p;
+            }
+            std::sort(key.begin(), key.end());
+            perm.resize((size_t)2 * Mp);
+            for (uint32_t p = 0; p < Mp; ++p)
+                perm[p] = p < n_snps ? (uint32_t)key[p] : p;
+            for (uint32_t p = 0; p < Mp; ++p)
+                perm[Mp + perm[p]] = p;
+            co->ordered = true;
+            h = co->host_desc();
+            e = hipMemcpyAsync(co->perm.ptr, perm.data(), perm.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, strm);
+        }
+        if (e == hipSuccess)
+            e = pack((const uint32_t *)co->perm.ptr);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm);
+        if (e == hipSuccess)
+            e = launch_singles((const CohortDev *)co->desc.ptr, co->Mp, strm);
+    }
     if (st == EPI_OK && e == hipSuccess && !co->blocked)
         e = launch_strips((const CohortDev *)co->desc.ptr, nullptr, nb, co->Mp, strm);
     if (st == EPI_OK && e == hipSuccess)
@@ -446,7 +495,7 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     co->bytes = 0;
     for (int c = 0; c < 2; ++c)
         co->bytes += co->sm[c].bytes + co->wm[c].bytes + co->pair[c].bytes + co->single[c].bytes;
-    co->bytes += co->desc.bytes + co->pslot.bytes + co->todo.bytes;
+    co->bytes += co->desc.bytes + co->pslot.bytes + co->todo.bytes + co->perm.bytes;
     *out = co;
     return EPI_OK;
 }
@@ -495,6 +544,8 @@ CohortDev epi_cohort::host_desc() const
     h.M = M;
     h.Mp = Mp;
     h.pslot = (const uint32_t *)pslot.ptr;
+    h.orig = ordered ? (const uint32_t *)perm.ptr : nullptr;
+    h.pos = ordered ? (const uint32_t *)perm.ptr + Mp : nullptr;
     // class 1's planes as a byte offset from class 0's, when it fits 32 bits
     const size_t e1 = (size_t)Mp * 2 * EL, f1 = (size_t)Mp * 2 * FL;
     h.E1off = e1 < ((size_t)1 << 32) ? (uint32_t)e1 : 0;
@@ -530,6 +581,15 @@ extern "C" int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes)
         return EPI_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->pair_budget = bytes ? bytes : ctx->pair_budget_default;
+    return EPI_OK;
+}
+
+extern "C" int epi_set_snp_order(epi_ctx *ctx, int mode)
+{
+    if (!ctx || mode < 0 || mode > 2)
+        return EPI_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->snp_order = mode;
     return EPI_OK;
 }
 

@@ -13,6 +13,7 @@ constexpr int kWaveCap = 128;  // per-wavefront LDS candidate buffer
 constexpr int kMfmaWaves = 8;  // wavefronts per workgroup of the MFMA scan kernel (2 per SIMD)
 constexpr uint32_t kMaxSamples = 1u << 22;    // n_samples below this (exact FP4 sums)
 constexpr uint32_t kMaxSnps = (1u << 21) - 1;  // n_snps at most this (21-bit key fields)
+constexpr uint32_t kOrderMinSnps = 1536;  // allele-frequency SNP order from here on (by default)
 
 // Candidate: FP32 score and a lexicographic key (i << 42 | j << 21 | k).
 struct Cand {
@@ -53,6 +54,11 @@ struct CohortDev {
     uint32_t n[2];        // class sizes
     uint32_t M, Mp;       // SNPs, padded to a multiple of 64
     const uint32_t *pslot;  // [Mp / 64] strip slot of each column block
+    // Stored SNP order: row p of every array above holds original SNP
+    // orig[p] (p < M; orig[p] = p for the padding rows), and original SNP s
+    // is stored at row pos[s].  Both null: rows are in original order.
+    const uint32_t *orig;   // [Mp]
+    const uint32_t *pos;    // [Mp]
 };
 
 // Offset of pair (p, q) in a class's strips, given the slot of q's block.
@@ -129,8 +135,9 @@ struct ScanConfig {
 
 // Launch wrappers (episdet_kernels.hip).
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+// Row p of the planes holds genotype column src[p] (src null: column p).
 hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
-                       const uint32_t *order, uint32_t n0, uint32_t n1,
+                       const uint32_t *src, const uint32_t *order, uint32_t n0, uint32_t n1,
                        uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1,
                        uint32_t W0, uint32_t W1, hipStream_t s);
 hipError_t launch_bed_unpack(const uint8_t *bed, uint32_t N, uint32_t M, uint8_t *geno,

@@ -33,6 +33,7 @@
 
 #include "episdet_cells.h"
 #include "episdet_internal.h"
+#include "episdet_order.h"
 #include "episdet_topk.h"
 
 #define EPI_CONST __attribute__((address_space(4)))
@@ -107,13 +108,15 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s)
 
 // ------------------------------------------------------------------------
 // packing (GPU/v4.cpp:112-196): order[] lists controls then cases, each in
-// original sample order; sample t of class c goes to bit t%64 of word t/64.
+// original sample order; sample t of class c goes to bit t%64 of word t/64;
+// row p holds genotype column src[p] (src null: column p).
 // ------------------------------------------------------------------------
 // One-dimensional grid of (class, word, 256-SNP chunk) blocks, chunks
 // fastest (no grid dimension bounded by the word count).
 __global__ void __launch_bounds__(256)
 pack_kernel(const uint8_t *__restrict__ geno, uint32_t M, uint32_t Mp,
-            const uint32_t *__restrict__ order, uint32_t n0, uint32_t n1,
+            const uint32_t *__restrict__ src, const uint32_t *__restrict__ order,
+            uint32_t n0, uint32_t n1,
             uint4 *sm0, uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1,
             uint32_t chunks, uint32_t wmax)
 {
@@ -128,10 +131,11 @@ pack_kernel(const uint8_t *__restrict__ geno, uint32_t M, uint32_t Mp,
     const uint32_t *ord = order + (c ? n0 : 0);
     uint64_t g0 = 0, g1 = 0;
     if (s < M) {
+        const uint32_t col = src ? src[s] : s;
         const uint32_t t0 = w * 64;
         const uint32_t tn = (n - t0) < 64u ? (n - t0) : 64u;
         for (uint32_t t = 0; t < tn; ++t) {
-            const uint8_t g = geno[(size_t)ord[t0 + t] * M + s];
+            const uint8_t g = geno[(size_t)ord[t0 + t] * M + col];
             g0 |= (uint64_t)(g == 0) << t;
             g1 |= (uint64_t)(g == 1) << t;
         }
@@ -143,7 +147,8 @@ pack_kernel(const uint8_t *__restrict__ geno, uint32_t M, uint32_t Mp,
 }
 
 hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
-                       const uint32_t *order, uint32_t n0, uint32_t n1, uint4 *sm0,
+                       const uint32_t *src, const uint32_t *order, uint32_t n0, uint32_t n1,
+                       uint4 *sm0,
                        uint4 *wm0, uint4 *sm1, uint4 *wm1, uint32_t W0, uint32_t W1,
                        hipStream_t s)
 {
@@ -155,8 +160,8 @@ hipError_t launch_pack(const uint8_t *geno, uint32_t N, uint32_t M, uint32_t Mp,
     const uint64_t blocks = (uint64_t)chunks * wmax * 2;
     if (blocks > 0x7FFFFFFFull)
         return hipErrorInvalidConfiguration;  // callers bound the shape first
-    hipLaunchKernelGGL(pack_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, geno, M, Mp, order,
-                       n0, n1, sm0, wm0, sm1, wm1, W0, W1, chunks, wmax);
+    hipLaunchKernelGGL(pack_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, geno, M, Mp, src,
+                       order, n0, n1, sm0, wm0, sm1, wm1, W0, W1, chunks, wmax);
     return hipGetLastError();
 }
 
@@ -334,6 +339,41 @@ __device__ __forceinline__ void count8(const uint4 *X, const uint4 *Y,
     }
 }
 
+// The 27 cells (9 g_a + 3 g_b + g_c) of one class for SNP rows a, b, c
+// (per lane), counted word by word from the planes; genotype 2 is neither
+// plane, masked to the class's samples in the last word.
+__device__ __forceinline__ void recount27(const uint4 *S, uint32_t W, uint32_t n, uint32_t a,
+                                          uint32_t b, uint32_t c, uint32_t (&x)[27])
+{
+#pragma unroll
+    for (int m = 0; m < 27; ++m)
+        x[m] = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint32_t rem = n - w * 64;
+        const uint64_t live = rem >= 64 ? ~0ull : ((1ull << rem) - 1ull);
+        const uint4 pa = S[(size_t)a * W + w], pb = S[(size_t)b * W + w], pc = S[(size_t)c * W + w];
+        uint64_t ga[3], gb[3], gc[3];
+        ga[0] = pa.x | ((uint64_t)pa.y << 32);
+        ga[1] = pa.z | ((uint64_t)pa.w << 32);
+        ga[2] = ~(ga[0] | ga[1]) & live;
+        gb[0] = pb.x | ((uint64_t)pb.y << 32);
+        gb[1] = pb.z | ((uint64_t)pb.w << 32);
+        gb[2] = ~(gb[0] | gb[1]) & live;
+        gc[0] = pc.x | ((uint64_t)pc.y << 32);
+        gc[1] = pc.z | ((uint64_t)pc.w << 32);
+        gc[2] = ~(gc[0] | gc[1]) & live;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const uint64_t uv = ga[u] & gb[v];
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    x[9 * u + 3 * v + t] += __popcll(uv & gc[t]);
+            }
+    }
+}
+
 // LDS: per-wavefront candidate buffers, then (optionally) the ln(n!) table.
 template <int NW>
 struct ScanLds {
@@ -455,6 +495,7 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
             const uint32_t j = j0 + jj;
             const bool valid = (i < j) && (j < k) && (k < M);
             float s = INFINITY;
+            uint64_t key = make_key(i, j, k);
             if (valid) {
                 uint32_t x0[27], x1[27];
 #pragma unroll
@@ -492,20 +533,60 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
                         __builtin_amdgcn_sched_barrier(0);
                 }
                 s = fabsf(sc);
-                if (DUMP) {
-                    const uint64_t rk = rank3(M, i, j, k);
-                    P.dump_scores[rk] = s;
-                    if (P.dump_counts) {
+                // Rows in another order than the original one (C->orig):
+                // the key holds the original indices, and the reference sums
+                // the cells in the order of those.  The stored-order sum is
+                // within kOrderSlack of that sum, so only lanes that could
+                // reach the top list are looked at; those whose axis order
+                // is not the identity are counted again with their rows in
+                // original index order and summed in order.
+                uint32_t t3[3] = {i, j, k};
+                bool again = false;
+                if (C->orig && (DUMP || s * kOrderSlack <= fminf(W.gthr, W.ts))) {
+                    const uint32_t oi = C->orig[i], oj = C->orig[j], ok = C->orig[k];
+                    sort3(oi, oj, ok, t3);
+                    key = make_key(t3[0], t3[1], t3[2]);
+                    again = axis_order(oi, oj, ok) != 0;
+                }
+                if (again) {
+                    auto row = [&](uint32_t t) { return t == C->orig[i] ? i : t == C->orig[j] ? j : k; };
+                    const uint32_t ra = row(t3[0]), rb = row(t3[1]), rc = row(t3[2]);
+                    uint32_t y0[27], y1[27];
+                    recount27(C->sm[0], C->W[0], n0, ra, rb, rc, y0);
+                    recount27(C->sm[1], C->W[1], n1, ra, rb, rc, y1);
+                    float sr = 0.0f;
+#pragma unroll
+                    for (int m = 0; m < 27; ++m) {
+                        const uint32_t ia = min(y0[m] + y1[m] + 1, tmax);
+                        const uint32_t ib = min(y0[m], tmax);
+                        const uint32_t id = min(y1[m], tmax);
+                        const float a = TLDS ? Tl[ia] : P.table[ia];
+                        const float b = TLDS ? Tl[ib] : P.table[ib];
+                        const float d = TLDS ? Tl[id] : P.table[id];
+                        sr += (a - b) - d;
+                    }
+                    s = fabsf(sr);
+                    if (DUMP && P.dump_counts) {
+                        const uint64_t rk = rank3(M, t3[0], t3[1], t3[2]);
 #pragma unroll
                         for (int m = 0; m < 27; ++m) {
-                            P.dump_counts[rk * 54 + m] = x0[m];
-                            P.dump_counts[rk * 54 + 27 + m] = x1[m];
+                            P.dump_counts[rk * 54 + m] = y0[m];
+                            P.dump_counts[rk * 54 + 27 + m] = y1[m];
                         }
                     }
+                } else if (DUMP && P.dump_counts) {
+                    const uint64_t rk = rank3(M, t3[0], t3[1], t3[2]);
+#pragma unroll
+                    for (int m = 0; m < 27; ++m) {
+                        P.dump_counts[rk * 54 + m] = x0[m];
+                        P.dump_counts[rk * 54 + 27 + m] = x1[m];
+                    }
                 }
+                if (DUMP)
+                    P.dump_scores[rank3(M, t3[0], t3[1], t3[2])] = s;
             }
             if (!DUMP)
-                topk::wave_offer(W, valid, s, make_key(i, j, k), lane);
+                topk::wave_offer(W, valid, s, key, lane);
         }
     }
 }
@@ -772,6 +853,10 @@ rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ c
     }
     const uint32_t i = key_i(cd.key), j = key_j(cd.key), k = key_k(cd.key);
     const CohortDev &C = cohorts[cohort];
+    // the rows of SNPs i, j, k (in original index order, so the cells come
+    // out in the reference's order 9 g_i + 3 g_j + g_k)
+    const uint32_t ri = C.pos ? C.pos[i] : i, rj = C.pos ? C.pos[j] : j,
+                   rk = C.pos ? C.pos[k] : k;
     uint32_t cnt[54];
 #pragma unroll
     for (int m = 0; m < 54; ++m)
@@ -782,9 +867,9 @@ rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ c
         const uint32_t n = C.n[c];
         const uint4 *S = C.sm[c];
         for (uint32_t w = lane; w < W; w += 64) {
-            const uint4 a = S[(size_t)i * W + w];
-            const uint4 b = S[(size_t)j * W + w];
-            const uint4 d = S[(size_t)k * W + w];
+            const uint4 a = S[(size_t)ri * W + w];
+            const uint4 b = S[(size_t)rj * W + w];
+            const uint4 d = S[(size_t)rk * W + w];
             const uint32_t rem = n - w * 64;
             const uint64_t live = rem >= 64 ? ~0ull : ((1ull << rem) - 1ull);
             uint64_t x[3], y[3], z[3];

@@ -201,6 +201,10 @@ struct epi_ctx {
     // classes); a cohort whose full set would exceed it keeps a few strips
     // that its scans refill (epi_set_pair_budget)
     uint64_t pair_budget = 0, pair_budget_default = 0;
+    // stored SNP order of cohorts created from now on (epi_set_snp_order):
+    // 0 original, 1 allele frequency, 2 allele frequency from
+    // kOrderMinSnps SNPs on
+    int snp_order = 2;
     // ln(n!) tables (FP32 reference semantics, FP64 twin), table_len entries
     uint32_t table_len = 0;
     epi::DevBuf t32, t64;
@@ -247,6 +251,10 @@ struct epi_cohort {
     // the FP4 one (F); rows of one pitch (EL, FL bytes) in both classes
     epi::DevBuf E, F;
     uint32_t EL = 0, FL = 0;
+    // stored SNP order (ordered): perm holds orig[Mp] (row -> original SNP)
+    // then pos[Mp] (original SNP -> row)
+    epi::DevBuf perm;
+    bool ordered = false;
     // triplet-tile rows for tile width rows_jp (0: none yet), built on first use
     std::vector<epi::Row> rows;
     uint64_t rows_items = 0;

@@ -39,6 +39,7 @@
 
 #include "episdet_cells.h"
 #include "episdet_internal.h"
+#include "episdet_order.h"
 #include "episdet_topk.h"
 
 #define EPI_CONST __attribute__((address_space(4)))
@@ -236,6 +237,7 @@ struct MfMargins {
     uint4 ik[2][64];            // pair margins of (i, k)
     uint2 sj[2][64];            // single-SNP counts of the j-block
     uint2 sk[2][64];            // and of the k-block
+    uint32_t oj[64], ok[64];    // original indices of the j- and k-block SNPs
 };
 
 struct MfLds {
@@ -402,6 +404,8 @@ __device__ __forceinline__ void mf_stage_margins(MfMargins &G, const EPI_CONST C
         G.ij[c][e] = x4(x);
         G.sj[c][e] = make_uint2(y.x << 2, y.y << 2);
     }
+    if (c == 0)
+        (kb ? G.ok : G.oj)[e] = C->orig ? C->orig[q] : q;
 }
 
 // 4 x a count from an accumulator element: the FP4 sums are already 4 x
@@ -746,6 +750,7 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
 #pragma unroll
         for (int c = 0; c < 2; ++c)
             jkn[c] = jkp[c][0];
+        uint32_t redo_mask = 0;  // bit s8: the lane's slot s8 is summed again below
 #pragma unroll 1
         for (uint32_t s8 = 0; s8 < 8; ++s8) {
             uint4 jkc[2];
@@ -773,49 +778,46 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             // below is at most 4 (N + 1), inside the table.
             if (__ballot(valid) == 0)
                 continue;
-            uint32_t t8[2][8];  // [class][4a + 2b + c]
-            switch (s8) {
-#define EPI_MF_SLOT(S8)                                                                   \
+            // t8[class][4a + 2b + c]: the slot's 16 accumulators
+#define EPI_MF_T8(t8)                                                                     \
+    switch (s8) {                                                                         \
+        EPI_MF_SLOT(t8, 0)                                                                \
+        EPI_MF_SLOT(t8, 1)                                                                \
+        EPI_MF_SLOT(t8, 2)                                                                \
+        EPI_MF_SLOT(t8, 3)                                                                \
+        EPI_MF_SLOT(t8, 4)                                                                \
+        EPI_MF_SLOT(t8, 5)                                                                \
+        EPI_MF_SLOT(t8, 6)                                                                \
+        EPI_MF_SLOT(t8, 7)                                                                \
+    }
+#define EPI_MF_SLOT(t8, S8)                                                               \
     case S8:                                                                              \
         for (int c = 0; c < 2; ++c)                                                       \
             for (int ab = 0; ab < 4; ++ab)                                                \
                 for (int cc = 0; cc < 2; ++cc)                                            \
                     t8[c][2 * ab + cc] = acc_x4(acc[c][(S8) >> 2][cc][4 * ((S8) & 3) + ab]); \
         break;
-                EPI_MF_SLOT(0)
-                EPI_MF_SLOT(1)
-                EPI_MF_SLOT(2)
-                EPI_MF_SLOT(3)
-                EPI_MF_SLOT(4)
-                EPI_MF_SLOT(5)
-                EPI_MF_SLOT(6)
-                EPI_MF_SLOT(7)
-#undef EPI_MF_SLOT
-            }
+            // x[class][9a + 3b + c]: the slot's 27 cells (x 4) from t8 and
+            // the margins, (j, k) pair margins jkv
+#define EPI_MF_CELLS(x, t8, jkv)                                                          \
+    for (int c = 0; c < 2; ++c) {                                                         \
+        const uint4 ij = L.mg.ij[c][jl];                                                  \
+        const uint4 jk = jkv[c];                                                          \
+        const uint2 sj = L.mg.sj[c][jl];                                                  \
+        const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};                                 \
+        const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};                     \
+        const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};                                 \
+        const uint32_t asi[2] = {si[c].x, si[c].y};                                       \
+        const uint32_t asj[2] = {sj.x, sj.y};                                             \
+        const uint32_t ask[2] = {sk[c].x, sk[c].y};                                       \
+        derive27(t8[c], aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);               \
+    }
+            uint32_t t8[2][8];
+            EPI_MF_T8(t8)
             float sc = 0.0f;
             {
                 uint32_t x[2][27];
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const uint4 ij = L.mg.ij[c][jl];
-                    const uint4 jk = jkc[c];
-                    const uint2 sj = L.mg.sj[c][jl];
-                    const uint32_t aij[4] = {ij.x, ij.y, ij.z, ij.w};
-                    const uint32_t aik[4] = {ik[c].x, ik[c].y, ik[c].z, ik[c].w};
-                    const uint32_t ajk[4] = {jk.x, jk.y, jk.z, jk.w};
-                    const uint32_t asi[2] = {si[c].x, si[c].y};
-                    const uint32_t asj[2] = {sj.x, sj.y};
-                    const uint32_t ask[2] = {sk[c].x, sk[c].y};
-                    derive27(t8[c], aij, aik, ajk, asi, asj, ask, c ? n14 : n04, x[c]);
-                }
-                if (DUMP && P.dump_counts && valid) {
-                    const uint64_t rk = dump_rank(M, i, j, k);
-#pragma unroll
-                    for (int m = 0; m < 27; ++m) {
-                        P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
-                        P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
-                    }
-                }
+                EPI_MF_CELLS(x, t8, jkc)
                 // K2 in the reference order (GPU/v4.cpp:579-583,
                 // CPU/v2.cpp:516-519): ((T[n0 + n1 + 1] - T[n0]) - T[n1])
                 // summed over the cells in order from 0 (the first term
@@ -841,11 +843,109 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
                 }
             }
             const float s = valid ? fabsf(sc) : INFINITY;
-            if (DUMP && valid)
-                P.dump_scores[dump_rank(M, i, j, k)] = s;
+            // Lanes that may enter the top list: the stored-order sum is
+            // within kOrderSlack of the reference-order one, so a lane whose
+            // scaled sum exceeds the wave's and the cohort's thresholds
+            // cannot pass them in either order.  Most slots have none.
+            const bool near = valid && (DUMP || s * kOrderSlack <= fminf(Wt.gthr, Wt.ts));
+            if (__ballot(near) == 0)
+                continue;
+            // Rows in another order than the original one (C->orig): the key
+            // holds the sorted original indices, and the reference sums the
+            // cells in the order of those; a lane whose axis order is not the
+            // identity is summed again in that order after the slot loop
+            // (every dump lane goes there).
+            uint64_t key = make_key(i, j, k);
+            bool redo = DUMP && valid;
+            if (C->orig != nullptr) {
+                const uint32_t oi = C->orig[i], oj = L.mg.oj[jl], ok = L.mg.ok[kl];
+                uint32_t t3[3];
+                sort3(oi, oj, ok, t3);
+                key = make_key(t3[0], t3[1], t3[2]);
+                redo = redo || (near && axis_order(oi, oj, ok) != 0);
+            }
+            redo_mask |= (uint32_t)redo << s8;
             if (!DUMP)
-                topk::wave_offer(Wt, valid, s, make_key(i, j, k), lane);
+                topk::wave_offer(Wt, near && !redo, s, key, lane);
         }
+        // Lanes summed in the reference's cell order: the slot's cells are
+        // rebuilt from the accumulators and margins (the stored-order ones
+        // are not kept), with the axes in increasing original index.
+        if (__ballot(redo_mask != 0) != 0) {
+#pragma unroll 1
+            for (uint32_t s8 = 0; s8 < 8; ++s8) {
+                const bool rd = ((redo_mask >> s8) & 1u) != 0;
+                if (__ballot(rd) == 0)
+                    continue;
+                const uint32_t jl = jl0 + 32 * (s8 >> 2) + 8 * (s8 & 3);
+                const uint32_t j = J0 + jl;
+                uint32_t t8[2][8];
+                EPI_MF_T8(t8)
+                const size_t off = (size_t)(32 * (s8 >> 2) + 8 * (s8 & 3)) * 64;
+                uint32_t o = 0, t3[3] = {i, j, k};
+                if (C->orig != nullptr) {
+                    const uint32_t oi = C->orig[i], oj = L.mg.oj[jl], ok = L.mg.ok[kl];
+                    sort3(oi, oj, ok, t3);
+                    o = axis_order(oi, oj, ok);
+                }
+                // the cells in the reference's order: derive27 of the
+                // margins rearranged into the axes of increasing original index
+                uint32_t x[2][27];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const uint4 ij = L.mg.ij[c][jl];
+                    const uint4 jk = x4(jkp[c][off]);
+                    const uint2 sj = L.mg.sj[c][jl];
+                    Axes3 st, rf;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        st.t8[e] = t8[c][e];
+                    st.ij[0] = ij.x, st.ij[1] = ij.y, st.ij[2] = ij.z, st.ij[3] = ij.w;
+                    st.ik[0] = ik[c].x, st.ik[1] = ik[c].y, st.ik[2] = ik[c].z, st.ik[3] = ik[c].w;
+                    st.jk[0] = jk.x, st.jk[1] = jk.y, st.jk[2] = jk.z, st.jk[3] = jk.w;
+                    st.si[0] = si[c].x, st.si[1] = si[c].y;
+                    st.sj[0] = sj.x, st.sj[1] = sj.y;
+                    st.sk[0] = sk[c].x, st.sk[1] = sk[c].y;
+                    to_ref_axes(o, st, rf);
+                    derive27(rf.t8, rf.ij, rf.ik, rf.jk, rf.si, rf.sj, rf.sk, c ? n14 : n04, x[c]);
+                }
+                float sc = 0.0f;
+#pragma unroll
+                for (int m0 = 0; m0 < 27; m0 += kGather) {
+                    float ta[kGather], tb[kGather], td[kGather];
+#pragma unroll
+                    for (int m = 0; m < kGather; ++m) {
+                        const uint32_t ob = x[0][m0 + m], od = x[1][m0 + m];
+                        ta[m] = tab_at(tab + 4, ob + od);
+                        tb[m] = tab_at(tab, ob);
+                        td[m] = tab_at(tab, od);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int m = 0; m < kGather; ++m) {
+                        const float term = (ta[m] - tb[m]) - td[m];
+                        sc = (m0 + m == 0) ? term : sc + term;
+                    }
+                }
+                const float s = fabsf(sc);
+                if (DUMP && rd) {
+                    const uint64_t rk = dump_rank(M, t3[0], t3[1], t3[2]);
+                    P.dump_scores[rk] = s;
+                    if (P.dump_counts) {
+#pragma unroll
+                        for (int m = 0; m < 27; ++m) {
+                            P.dump_counts[rk * 54 + m] = x[0][m] >> 2;
+                            P.dump_counts[rk * 54 + 27 + m] = x[1][m] >> 2;
+                        }
+                    }
+                }
+                if (!DUMP)
+                    topk::wave_offer(Wt, rd, s, make_key(t3[0], t3[1], t3[2]), lane);
+            }
+        }
+#undef EPI_MF_CELLS
+#undef EPI_MF_SLOT
+#undef EPI_MF_T8
         // The wavefronts leave the epilogue together (measured faster than
         // letting the early ones run ahead into the next stage loop).
         __syncthreads();

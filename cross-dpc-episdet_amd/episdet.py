@@ -102,6 +102,7 @@ _PROTOS = [
     ("epi_cohort_bytes", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_cohort_footprint", ctypes.c_int, [_u32, _u32, ctypes.POINTER(ctypes.c_uint64)]),
     ("epi_set_pair_budget", ctypes.c_int, [_VP, ctypes.c_uint64]),
+    ("epi_set_snp_order", ctypes.c_int, [_VP, ctypes.c_int]),
     ("epi_scan", ctypes.c_int, [_VP, _VP, ctypes.POINTER(EpiTile), ctypes.POINTER(EpiResult)]),
     ("epi_scan_batch", ctypes.c_int, [_VP, ctypes.POINTER(_VP), _u32, ctypes.POINTER(EpiTile),
                                       ctypes.POINTER(EpiResult)]),
@@ -361,6 +362,12 @@ class Context:
         """Pair-margin bytes cohorts created afterwards keep resident (0:
         the default); past it a cohort is scanned in strip rounds."""
         _check(lib().epi_set_pair_budget(self.handle, int(nbytes)), "epi_set_pair_budget")
+
+    def set_snp_order(self, mode: int) -> None:
+        """Device SNP order of cohorts created afterwards: 1 allele
+        frequency, 0 original, 2 (the default) allele frequency from 1536
+        SNPs on.  Results are those of the original order either way."""
+        _check(lib().epi_set_snp_order(self.handle, int(mode)), "epi_set_snp_order")
 
     def scan_noalloc(self, cohort: "Cohort", tile=None) -> ScanResult:
         """epi_scan_noalloc: the same scan with no device allocation (a

@@ -134,6 +134,14 @@ int epi_cohort_footprint(uint32_t n_samples, uint32_t n_snps, uint64_t *bytes);
  * reference keeps no pair margins (its memory is O(n_snps x n_samples),
  * GPU/v4.cpp:256-257); no counterpart. */
 int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes);
+/* Device SNP order of cohorts created after this call: 1 allele
+ * frequency, which makes the scan's ln(n!) lookups cheaper on long scans;
+ * 0 the original order; 2 (the default) allele frequency for cohorts of
+ * at least 1536 SNPs, else original.  Every result (indices, scores,
+ * counts, tie order, dumps) is that of the original order bit for bit
+ * either way.  The reference keeps the input order (GPU/v4.cpp:112-196);
+ * no counterpart. */
+int epi_set_snp_order(epi_ctx *ctx, int mode);
 
 /* Scans ---------------------------------------------------------------- */
 /* Exhaustive scan of all C(M,3) triplets.  Replaces the block-triple

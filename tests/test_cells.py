@@ -45,3 +45,59 @@ def test_derive27_matches_per_sample_counting(derive, n, maf):
         x = derive(*margins(g, i, j, k), n)
         cell = 9 * g[:, i] + 3 * g[:, j] + g[:, k]
         assert np.array_equal(x, np.bincount(cell, minlength=27)), (i, j, k)
+
+
+# ---------------------------------------------------------------------------
+# Stored SNP order (episdet_order.h): a cohort may store its SNPs in another
+# order than the original one; a stored triplet p < q < r then has original
+# indices in one of six orders, and the reference's cells (axes by increasing
+# original index) must come out of the stored-axis inputs.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def order_lib():
+    if not os.path.exists(SO):
+        pytest.skip("tests/native/libcells_host.so not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(SO)
+    P = ctypes.POINTER(ctypes.c_uint32)
+    L.order_axis.restype = ctypes.c_uint32
+    L.order_axis.argtypes = [ctypes.c_uint32] * 3
+    L.order_stored_cell.restype = ctypes.c_int
+    L.order_stored_cell.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.order_ref_derive.argtypes = [ctypes.c_uint32, P, P, P, P, P, P, P, ctypes.c_uint32, P]
+    return L
+
+
+def test_axis_order_is_the_rank_code(order_lib):
+    import itertools
+    seen = set()
+    for perm in itertools.permutations([5, 17, 40]):
+        o = order_lib.order_axis(*perm)
+        ranks = [sorted(perm).index(v) for v in perm]
+        assert o == 2 * ranks[0] + (ranks[1] > ranks[2]), perm
+        seen.add(o)
+    assert seen == set(range(6))
+    assert order_lib.order_axis(1, 2, 3) == 0
+
+
+@pytest.mark.parametrize("n,seed", [(0, 1), (1, 2), (63, 3), (64, 4), (700, 5), (4097, 6)])
+def test_reference_axes_from_stored_axes(order_lib, n, seed):
+    """Cells of a stored triplet rebuilt in the reference's axis order equal
+    per-sample counting with the SNPs taken by increasing original index, for
+    all six orders; stored_cell maps the stored cells to the same table."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    q = rng.uniform(0.05, 0.5, 3)
+    g = np.stack([rng.choice(3, size=n, p=[(1 - a) ** 2, 2 * a * (1 - a), a * a]) for a in q], 1)
+    P = ctypes.POINTER(ctypes.c_uint32)
+    stored = np.bincount(9 * g[:, 0] + 3 * g[:, 1] + g[:, 2], minlength=27)
+    inputs = [np.ascontiguousarray(a, np.uint32) for a in margins(g, 0, 1, 2)]
+    for orig in itertools.permutations([3, 11, 29]):
+        o = order_lib.order_axis(*orig)
+        ax = np.argsort(orig)  # stored axes by increasing original index
+        ref = np.bincount(9 * g[:, ax[0]] + 3 * g[:, ax[1]] + g[:, ax[2]], minlength=27)
+        x = np.zeros(27, np.uint32)
+        order_lib.order_ref_derive(o, *[a.ctypes.data_as(P) for a in inputs], n,
+                                   x.ctypes.data_as(P))
+        assert np.array_equal(x, ref), (orig, o)
+        sc = [order_lib.order_stored_cell(o, m) for m in range(27)]
+        assert np.array_equal(stored[sc], ref), (orig, o)
