@@ -444,9 +444,12 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     // Lanes that may reach the top list and whose SNPs are out of original
     // order are summed twice, which costs more than the order saves until
     // each wavefront's threshold has settled: below kOrderMinSnps SNPs the
-    // scan is too short for that (by default the order is used from there).
+    // scan is too short for that, and below kOrderMinSamples samples the
+    // counts are small and the lookups conflict little in either order (by
+    // default the order is used from there).
     std::vector<uint32_t> perm;
-    const bool ordered = ctx->snp_order == 1 || (ctx->snp_order == 2 && n_snps >= kOrderMinSnps);
+    const bool ordered = ctx->snp_order == 1 ||
+                         (ctx->snp_order == 2 && n_snps >= kOrderMinSnps && kept >= kOrderMinSamples);
     if (st == EPI_OK && e == hipSuccess && ordered)
         st = co->perm.reserve((size_t)2 * co->Mp * sizeof(uint32_t));
     if (st == EPI_OK && e == hipSuccess && ordered) {

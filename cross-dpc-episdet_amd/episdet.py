@@ -365,8 +365,8 @@ class Context:
 
     def set_snp_order(self, mode: int) -> None:
         """Device SNP order of cohorts created afterwards: 1 allele
-        frequency, 0 original, 2 (the default) allele frequency from 1536
-        SNPs on.  Results are those of the original order either way."""
+        frequency, 0 original, 2 (the default) allele frequency from 1024
+        SNPs and 2048 samples on.  Results are those of the original order either way."""
         _check(lib().epi_set_snp_order(self.handle, int(mode)), "epi_set_snp_order")
 
     def scan_noalloc(self, cohort: "Cohort", tile=None) -> ScanResult:
