@@ -123,6 +123,9 @@ int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);
 void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
                 std::vector<Row> &rows, uint64_t &items);
+// Puts the rows (one i per item) of a cohort in allele-frequency SNP order
+// into a fixed pseudo-random order and renumbers their items.
+void shuffle_rows(std::vector<Row> &rows);
 // Round tables in one device buffer: [hdr][segments][descs][lists], sized
 // for `cap` cohorts.
 struct RoundLayout {

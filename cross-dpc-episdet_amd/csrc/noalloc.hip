@@ -120,6 +120,8 @@ extern "C" int epi_scan_noalloc(epi_ctx *ctx, epi_cohort *co, const epi_tile *ti
     build_rows(0, co->M, co->Mp, cfg.snps_j, 1, job.rows, job.items);
     if (job.items >= 0x7FFFFFFFull)
         return EPI_EINVAL;
+    if (co->ordered)
+        shuffle_rows(job.rows);
     std::vector<float> t32;
     std::vector<double> t64;
     build_log_tables(table_len - 1, t32, t64);

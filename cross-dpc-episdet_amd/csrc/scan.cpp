@@ -82,6 +82,29 @@ void build_rows(uint32_t cohort, uint32_t M, uint32_t Mp, int jp, int nw,
     }
 }
 
+// With the SNPs in allele-frequency order, rows in k-block order would
+// visit the triplets by allele frequency of their last SNP; where the best
+// scores sit at one end, the wavefronts' thresholds would trail the scan
+// and far more lanes would pass them (and be summed twice).  A fixed
+// pseudo-random row order (xorshift64 Fisher-Yates) decorrelates them; each
+// row still runs its items together, so its j and k blocks stay in cache.
+// Rows of one i per item (nw = 1).
+void shuffle_rows(std::vector<Row> &rows)
+{
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (size_t t = rows.size(); t > 1; --t) {
+        h ^= h << 13;
+        h ^= h >> 7;
+        h ^= h << 17;
+        std::swap(rows[t - 1], rows[h % t]);
+    }
+    uint32_t it = 0;
+    for (Row &r : rows) {
+        r.item_start = it;
+        it += r.icount;
+    }
+}
+
 // Kernel and tile shape: variant 2 or 3 in `tile` selects that MFMA kernel
 // (int8 or FP4); a tile naming the VALU variant or a VALU shape selects the
 // VALU kernel; otherwise the context's tile table (per sample-count bucket)
@@ -176,27 +199,8 @@ static int cohort_rows(epi_ctx *ctx, epi_cohort *co, int jp)
     build_rows(0, co->M, co->Mp, jp, 1, rows, items);
     if (items >= 0x7FFFFFFFull)
         return EPI_EINVAL;
-    if (co->ordered) {
-        // With the SNPs in allele-frequency order, rows in k-block order
-        // would visit the triplets by allele frequency of their last SNP;
-        // where the best scores sit at one end, the wavefronts' thresholds
-        // would trail the scan and far more lanes would pass them (and be
-        // summed twice).  A fixed pseudo-random row order (xorshift64
-        // Fisher-Yates) decorrelates them; each row still runs its items
-        // together, so its j and k blocks stay in cache.
-        uint64_t h = 0x9E3779B97F4A7C15ull;
-        for (size_t t = rows.size(); t > 1; --t) {
-            h ^= h << 13;
-            h ^= h >> 7;
-            h ^= h << 17;
-            std::swap(rows[t - 1], rows[h % t]);
-        }
-        uint32_t it = 0;
-        for (Row &r : rows) {
-            r.item_start = it;
-            it += r.icount;
-        }
-    }
+    if (co->ordered)
+        shuffle_rows(rows);
     EPI_TRY(co->drows.reserve(sizeof(Row) * std::max<size_t>(rows.size(), 1)));
     EPI_TRY(stage_upload(ctx, co->drows.ptr, rows.data(), sizeof(Row) * rows.size()));
     co->rows = std::move(rows);

@@ -153,3 +153,17 @@ def test_strip_rounds_on_ordered_cohort(epi, plain_ctx, var):
     with plain_ctx.cohort(g, p) as co:
         b = plain_ctx.scan(co, tile={"variant": var})
     assert same(a, b)
+
+
+@pytest.mark.parametrize("var", VARIANTS)
+def test_noalloc_on_ordered_cohort(epi, ord_ctx, plain_ctx, var):
+    """epi_scan_noalloc of a cohort in stored order (its rows shuffled as
+    epi_scan's are) equals the original-order scan."""
+    g, p = epi.generate_hwe(36, 1800, 300, 800)
+    say("order noalloc", var)
+    with ord_ctx.cohort(g, p) as co:
+        ord_ctx.scan(co, tile={"variant": var})  # builds the expanded planes
+        a = ord_ctx.scan_noalloc(co, tile={"variant": var})
+    with plain_ctx.cohort(g, p) as co:
+        b = plain_ctx.scan(co, tile={"variant": var})
+    assert same(a, b)
