@@ -167,3 +167,23 @@ def test_noalloc_on_ordered_cohort(epi, ord_ctx, plain_ctx, var):
     with plain_ctx.cohort(g, p) as co:
         b = plain_ctx.scan(co, tile={"variant": var})
     assert same(a, b)
+
+
+def test_bed_cohort_in_stored_order(epi, orc, ord_ctx, plain_ctx):
+    """A .bed cohort (unpacked on the device) in stored order: scan and
+    epi_count equal the sample-major cohort in original order."""
+    N, M = 1500, 200
+    geno, pheno = epi.generate_hwe(37, N, M, N // 2)
+    rng = np.random.default_rng(37)
+    geno[rng.random(geno.shape) < 0.02] = 3  # missing calls count as genotype 2
+    bed = orc.bed_encode(geno)
+    say("order bed")
+    tr = np.array([[0, 1, 2], [3, 17, M - 1], [M - 3, M - 2, M - 1], [5, 100, 150]], np.uint32)
+    with ord_ctx.cohort_bed(bed, pheno, N) as b:
+        rb = ord_ctx.scan(b)
+        cb, sb, _ = ord_ctx.count(b, tr)
+    with plain_ctx.cohort(geno, pheno) as a:
+        ra = plain_ctx.scan(a)
+        ca, sa, _ = plain_ctx.count(a, tr)
+    assert same(ra, rb)
+    assert np.array_equal(ca, cb) and np.array_equal(sa, sb)
