@@ -448,8 +448,9 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     // counts are small and the lookups conflict little in either order (by
     // default the order is used from there).
     std::vector<uint32_t> perm;
-    const bool ordered = ctx->snp_order == 1 ||
-                         (ctx->snp_order == 2 && n_snps >= kOrderMinSnps && kept >= kOrderMinSamples);
+    const bool ordered =
+        ctx->snp_order == 1 ||
+        (ctx->snp_order == 2 && n_snps >= kOrderMinSnps && kept >= kOrderMinSamples);
     if (st == EPI_OK && e == hipSuccess && ordered)
         st = co->perm.reserve((size_t)2 * co->Mp * sizeof(uint32_t));
     if (st == EPI_OK && e == hipSuccess && ordered) {

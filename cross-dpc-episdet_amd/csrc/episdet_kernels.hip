@@ -549,7 +549,9 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
                     again = axis_order(oi, oj, ok) != 0;
                 }
                 if (again) {
-                    auto row = [&](uint32_t t) { return t == C->orig[i] ? i : t == C->orig[j] ? j : k; };
+                    auto row = [&](uint32_t t) {
+                        return t == C->orig[i] ? i : t == C->orig[j] ? j : k;
+                    };
                     const uint32_t ra = row(t3[0]), rb = row(t3[1]), rc = row(t3[2]);
                     uint32_t y0[27], y1[27];
                     recount27(C->sm[0], C->W[0], n0, ra, rb, rc, y0);
