@@ -377,7 +377,7 @@ __device__ __forceinline__ void recount27(const uint4 *S, uint32_t W, uint32_t n
 // LDS: per-wavefront candidate buffers, then (optionally) the ln(n!) table.
 template <int NW>
 struct ScanLds {
-    Cand cand[NW][kWaveCap + kTop];
+    Cand cand[NW][kWaveCap];
 };
 
 template <int NW>
@@ -401,7 +401,6 @@ __global__ void __launch_bounds__(NW * 64, 4) scan_kernel(const ScanParams P)
     const uint32_t wave = rfl(tid >> 6);
     topk::WaveTop W;
     W.buf = L.cand[wave];
-    W.keep = W.buf + kWaveCap;
     W.reset();
 
     if (TLDS) {

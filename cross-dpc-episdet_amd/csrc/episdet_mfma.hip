@@ -241,8 +241,9 @@ struct MfMargins {
 };
 
 struct MfLds {
-    Cand cand[kMfWaves][kWaveCap + kTop];
+    Cand cand[kMfWaves][kWaveCap];
     uint32_t misc[4];
+    uint32_t flush[kMfWaves + 1];  // a flush's per-wavefront counts, then its base
     MfMargins mg;      // the epilogue's margins
     MfStage stage[2];  // double buffer: one is read while the other is filled
 };
@@ -438,7 +439,6 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
     const uint32_t wave = mf::rfl(tid >> 6);
     topk::WaveTop Wt;
     Wt.buf = L.cand[wave];
-    Wt.keep = Wt.buf + kWaveCap;
     Wt.reset();
 
     if (TLDS)
@@ -525,8 +525,30 @@ __global__ void __launch_bounds__(kMfWaves * 64, 1) mfma_scan_kernel(const ScanP
             cohort = segs[sg].cohort;
         }
         if (cohort != cur) {
-            if (!DUMP && cur != 0xFFFFFFFFu)
-                topk::wave_flush(Wt, P, cur, lane);
+            if (!DUMP && cur != 0xFFFFFFFFu) {
+                // The workgroup's wavefronts leave a cohort together (they
+                // take the same items), so they append with one atomic: at
+                // the end of a scan every workgroup flushes at once, and one
+                // counter per cohort taken by every wavefront serializes.
+                bool pass;
+                const uint32_t nn = topk::flush_select(Wt, P, cur, lane, pass);
+                if (lane == 0)
+                    L.flush[wave] = nn;
+                __syncthreads();
+                if (tid == 0) {
+                    uint32_t tot = 0;
+#pragma unroll
+                    for (int w = 0; w < kMfWaves; ++w)
+                        tot += L.flush[w];
+                    L.flush[kMfWaves] = tot ? atomicAdd(&P.counters[1 + cur], tot) : 0u;
+                }
+                __syncthreads();
+                uint32_t base = L.flush[kMfWaves];
+                for (uint32_t w = 0; w < wave; ++w)
+                    base += L.flush[w];
+                topk::flush_append(Wt, P, cur, lane, pass, mf::rfl(base), nn);
+                Wt.reset();
+            }
             cur = cohort;
         }
         if (item >= n_items)
