@@ -211,7 +211,7 @@ constexpr int kRowBytes = kStage;     // LDS bytes per staged (snp, plane) row
 constexpr int kStageRows = 64 * 2;    // 64 SNPs x 2 planes
 constexpr int kStageCopies = 2 * kStageRows * kRowBytes / 1024 / kMfWaves;  // 1 KiB copies per wave
 constexpr int kIChunks = 2 * kStage / 16 / kMfWaves;  // 16-byte chunks of SNP i's rows per wave
-constexpr int kGather = 9;  // K2 cells whose table reads are issued ahead of their sums (divides 27)
+constexpr int kGather = 3;  // K2 cells whose table reads are issued ahead of their sums (divides 27)
 static_assert(kStageCopies == 4 && kIChunks == 2, "copy split");
 
 // Staged rows are 128 bytes with no padding; 16-byte slot s of row r holds
