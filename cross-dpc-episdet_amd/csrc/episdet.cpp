@@ -512,7 +512,7 @@ extern "C" int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_
     if (!ctx || !geno || !pheno || !out || !epi::cohort_shape_ok(n_samples, n_snps))
         return EPI_EINVAL;
     *out = nullptr;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     return epi::cohort_create_locked(ctx, geno, nullptr, pheno, n_samples, n_snps, out,
                                      ctx->stream);
 }
@@ -524,7 +524,7 @@ extern "C" int epi_cohort_create_bed(epi_ctx *ctx, const uint8_t *bed, const uin
         *out = nullptr;
     if (!ctx || !bed || !pheno || !out || !epi::cohort_shape_ok(n_samples, n_snps))
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     return epi::cohort_create_locked(ctx, nullptr, bed, pheno, n_samples, n_snps, out,
                                      ctx->stream);
 }
@@ -562,7 +562,7 @@ extern "C" int epi_cohort_destroy(epi_cohort *co)
     if (!co)
         return EPI_EINVAL;
     epi_ctx *ctx = co->ctx;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     if (co->server_refs != 0)
         return EPI_EBUSY;  // a server request still scans it
     (void)hipSetDevice(ctx->device);
@@ -583,7 +583,7 @@ extern "C" int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes)
 {
     if (!ctx)
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     ctx->pair_budget = bytes ? bytes : ctx->pair_budget_default;
     return EPI_OK;
 }
@@ -592,7 +592,7 @@ extern "C" int epi_set_snp_order(epi_ctx *ctx, int mode)
 {
     if (!ctx || mode < 0 || mode > 2)
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     ctx->snp_order = mode;
     return EPI_OK;
 }

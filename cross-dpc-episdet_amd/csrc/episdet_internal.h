@@ -170,8 +170,8 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s);
 // Folds the workgroup slots of each cohort of the round into its running
 // top-kTop list lists[c] (read and written).
 hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
-                        uint32_t slot_cap, Cand *const *lists, uint32_t max_cohorts,
-                        hipStream_t s);
+                        const uint32_t *thr, uint32_t slot_cap, Cand *const *lists,
+                        uint32_t max_cohorts, hipStream_t s);
 hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *thr,
                            uint32_t max_cohorts, hipStream_t s);
 // Rescores per_cohort candidates of each cohort: cands given either as one

@@ -713,91 +713,166 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-// Bitonic sort of the 1024 entries of `ent` by (score, key), one entry per
-// thread of the 1024-thread workgroup (55 compare-exchange steps).
-__device__ void sort1024(Cand *ent, uint32_t t)
+// A candidate held by one lane: score and the two halves of the key, each
+// moved between lanes on its own.
+struct LaneCand {
+    float s;
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ bool lc_less(LaneCand a, LaneCand b)
 {
-    for (uint32_t k = 2; k <= 1024; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t q = t ^ j;
-            if (q > t) {
-                const Cand a = ent[t], b = ent[q];
-                const bool up = (t & k) == 0;
-                if (up ? cand_less(b.s, b.key, a.s, a.key) : cand_less(a.s, a.key, b.s, b.key)) {
-                    ent[t] = b;
-                    ent[q] = a;
-                }
-            }
-            __syncthreads();
-        }
+    // no short-circuit: plain compares and selects, no branches
+    const uint64_t ka = ((uint64_t)a.hi << 32) | a.lo, kb = ((uint64_t)b.hi << 32) | b.lo;
+    return (a.s < b.s) | ((a.s == b.s) & (ka < kb));
+}
+__device__ __forceinline__ LaneCand lc_of(const Cand &c)
+{
+    return {c.s, (uint32_t)c.key, (uint32_t)(c.key >> 32)};
+}
+__device__ __forceinline__ Cand lc_cand(const LaneCand &v)
+{
+    Cand c;
+    c.s = v.s;
+    c.pad = 0;
+    c.key = ((uint64_t)v.hi << 32) | v.lo;
+    return c;
+}
+__device__ __forceinline__ LaneCand lc_empty() { return {INFINITY, ~0u, ~0u}; }
+// c ? a : b field by field (a select of the whole struct goes through scratch)
+__device__ __forceinline__ LaneCand lc_sel(bool c, LaneCand a, LaneCand b)
+{
+    return {c ? a.s : b.s, c ? a.lo : b.lo, c ? a.hi : b.hi};
+}
+__device__ __forceinline__ LaneCand lc_xor(const LaneCand &v, int m)
+{
+    return {__shfl_xor(v.s, m), (uint32_t)__shfl_xor((int)v.lo, m),
+            (uint32_t)__shfl_xor((int)v.hi, m)};
+}
+__device__ __forceinline__ LaneCand lc_from(const LaneCand &v, int src)
+{
+    return {__shfl(v.s, src), (uint32_t)__shfl((int)v.lo, src), (uint32_t)__shfl((int)v.hi, src)};
+}
+// lane b's entry (b wave-uniform)
+__device__ __forceinline__ LaneCand lc_read(const LaneCand &v, int b)
+{
+    return {__uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v.s), b)),
+            (uint32_t)__builtin_amdgcn_readlane((int)v.lo, b),
+            (uint32_t)__builtin_amdgcn_readlane((int)v.hi, b)};
 }
 
-// One workgroup per cohort: folds the entries the wavefronts appended into
-// the cohort's running list (sorted, the first 64 of `ent`).  The entries
-// are read 512 at a time (the next 512 loaded while these are filtered);
-// only those better than the current 64th best are appended behind the
-// list, and when the 1024 slots would overflow they are sorted and the best
-// 64 kept.  Most of the tens of thousands of appended entries lose to the
-// 64th best once a few hundred have been seen, so few sorts run.
+// One compare-exchange step of a bitonic network over the 64 lanes: lanes
+// `lane` and `lane ^ j` swap unless the lower one holds the smaller entry
+// (asc) or the larger (!asc).
+__device__ __forceinline__ void lc_step(LaneCand &v, uint32_t lane, int j, bool asc)
+{
+    const LaneCand p = lc_xor(v, j);
+    const bool want_min = ((lane & (uint32_t)j) == 0) == asc;
+    const bool p_less = lc_less(p, v), v_less = lc_less(v, p);
+    v = lc_sel(want_min ? p_less : v_less, p, v);
+}
+// Sorts the wavefront's 64 entries, lane 0 first: ascending, or descending.
+__device__ __forceinline__ void lc_sort(LaneCand &v, uint32_t lane, bool desc)
+{
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+            lc_step(v, lane, j, ((lane & (uint32_t)k) == 0) != desc);
+}
+// a ascending, d descending: the 64 best of both, ascending.
+__device__ __forceinline__ LaneCand lc_merge(const LaneCand &a, const LaneCand &d, uint32_t lane)
+{
+    LaneCand v = lc_sel(lc_less(d, a), d, a);  // bitonic, holding the 64 best
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1)
+        lc_step(v, lane, j, true);
+    return v;
+}
+
+// One workgroup of 16 wavefronts per cohort: folds the entries the scan
+// appended into the cohort's running list.  Each wavefront keeps a sorted
+// list of its 64 best in registers (lane l holds the l-th; wavefront 0
+// starts from the running list) and takes the entries 64 at a time; an
+// entry counts only if it beats both the wavefront's 64th best and the
+// workgroup's bound `bthr`, the smallest 64th-best score any of them holds
+// (or the cohort threshold the scan left: 64 distinct triplets at or below
+// it were appended or are in the running list).  A few such entries are
+// inserted one by one (rank by a ballot, the rest shift up a lane), more
+// are sorted and merged in (bitonic, no barriers).  Then the 16 lists are
+// merged pairwise in LDS, four levels.
 __global__ void __launch_bounds__(1024)
 merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
-             const uint32_t *__restrict__ counters, uint32_t cap, Cand *const *lists)
+             const uint32_t *__restrict__ counters, const uint32_t *__restrict__ thr,
+             uint32_t cap, Cand *const *lists)
 {
-    __shared__ Cand ent[1024];
-    __shared__ uint32_t fill;
+    constexpr int kWaves = 16, kInsertMax = 8;
+    __shared__ Cand part[kWaves][kTop];
+    __shared__ uint32_t bthr;
     const uint32_t cohort = blockIdx.x;
     if (cohort >= hdr->n_cohorts)
         return;
     const uint32_t t = threadIdx.x;
     const uint32_t lane = t & 63u;
+    const uint32_t wave = t >> 6;
     uint32_t n = counters[1 + cohort];
     if (n > cap)
         n = cap;
     const Cand *src = slots + (size_t)cohort * cap;
     Cand *list = lists[cohort];
-    Cand empty;
-    empty.s = INFINITY;
-    empty.pad = 0;
-    empty.key = kEmptyKey;
-    ent[t] = t < (uint32_t)kTop ? list[t] : empty;
     if (t == 0)
-        fill = kTop;
-    constexpr uint32_t step = 512;
-    Cand nxt = t < step && t < n ? src[t] : empty;
+        bthr = thr[cohort];
+    LaneCand L = lc_sel(wave == 0, lc_of(list[lane]), lc_empty());
+    LaneCand L63 = lc_read(L, 63);
     __syncthreads();
-    for (uint32_t base = 0; base < n; base += step) {
-        const Cand c = nxt;
-        nxt = t < step && base + step + t < n ? src[base + step + t] : empty;
-        const Cand thr = ent[kTop - 1];
-        bool keep = c.key != kEmptyKey && cand_less(c.s, c.key, thr.s, thr.key);
-        const uint32_t cnt = (uint32_t)__syncthreads_count(keep);
-        if (fill + cnt > 1024u) {
-            // make room: the best 64 so far, then filter again against them
-            sort1024(ent, t);
-            if (t >= (uint32_t)kTop)
-                ent[t] = empty;
-            if (t == 0)
-                fill = kTop;
-            __syncthreads();
-            const Cand thr2 = ent[kTop - 1];
-            keep = keep && cand_less(c.s, c.key, thr2.s, thr2.key);
+    const uint32_t chunks = (n + 63) / 64;
+    uint32_t c = wave;
+    LaneCand e = lc_empty();
+    if (c * 64 + lane < n)
+        e = lc_of(src[c * 64 + lane]);
+    for (; c < chunks; c += kWaves) {
+        const LaneCand cur = e;
+        const uint32_t x = (c + kWaves) * 64 + lane;
+        e = lc_empty();
+        if (x < n)
+            e = lc_of(src[x]);
+        const float tb = __uint_as_float(__atomic_load_n(&bthr, __ATOMIC_RELAXED));
+        const bool pass = cur.s <= tb && lc_less(cur, L63);
+        uint64_t m = __ballot(pass);
+        if (m == 0)
+            continue;
+        if (__popcll(m) <= kInsertMax) {
+            while (m) {
+                const int b = __builtin_ctzll(m);
+                m &= m - 1;
+                const LaneCand E = lc_read(cur, b);
+                if (!lc_less(E, L63))
+                    continue;
+                const uint32_t pos = (uint32_t)__popcll(__ballot(lc_less(L, E)));
+                const LaneCand up = lc_from(L, (int)lane - 1);
+                L = lc_sel(lane > pos, up, lc_sel(lane == pos, E, L));
+                L63 = lc_read(L, 63);
+            }
+        } else {
+            LaneCand d = lc_sel(pass, cur, lc_empty());
+            lc_sort(d, lane, true);
+            L = lc_merge(L, d, lane);
+            L63 = lc_read(L, 63);
         }
-        const uint64_t m = __ballot(keep);
-        uint32_t wbase = 0;
-        if (lane == 0 && m)
-            wbase = atomicAdd(&fill, (uint32_t)__popcll(m));
-        wbase = __shfl(wbase, 0);
-        if (keep) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            ent[wbase + below] = c;
+        if (lane == 0 && L63.s < tb)
+            atomicMin(&bthr, __float_as_uint(L63.s));
+    }
+    part[wave][lane] = lc_cand(L);
+    __syncthreads();
+    for (uint32_t s = 1; s < (uint32_t)kWaves; s <<= 1) {
+        if ((wave & (2 * s - 1)) == 0) {
+            const LaneCand a = lc_of(part[wave][lane]);
+            const LaneCand d = lc_of(part[wave + s][kTop - 1 - lane]);
+            part[wave][lane] = lc_cand(lc_merge(a, d, lane));
         }
         __syncthreads();
     }
-    if (fill > (uint32_t)kTop)
-        sort1024(ent, t);
-    if (t < (uint32_t)kTop)
-        list[t] = ent[t];
+    if (wave == 0)
+        list[lane] = part[0][lane];
 }
 
 // Per-launch cohort thresholds from the running lists (64th best score;
@@ -822,13 +897,13 @@ hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *th
 }
 
 hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
-                        uint32_t slot_cap, Cand *const *lists, uint32_t max_cohorts,
-                        hipStream_t s)
+                        const uint32_t *thr, uint32_t slot_cap, Cand *const *lists,
+                        uint32_t max_cohorts, hipStream_t s)
 {
     if (max_cohorts == 0)
         return hipSuccess;
     hipLaunchKernelGGL(merge_kernel, dim3(max_cohorts), dim3(1024), 0, s, hdr, slots, counters,
-                       slot_cap, lists);
+                       thr, slot_cap, lists);
     return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -59,6 +60,29 @@ struct DevBuf {
     ~DevBuf();
     int reserve(size_t want);
     void release();
+};
+
+// A lock taken in arrival order (a ticket lock): the server's worker, which
+// takes the context lock again right after each round, then queues behind
+// calls that were already waiting instead of starving them.
+struct FairMutex {
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t next = 0, serving = 0;
+    void lock()
+    {
+        std::unique_lock<std::mutex> lk(m);
+        const uint64_t mine = next++;
+        cv.wait(lk, [&] { return serving == mine; });
+    }
+    void unlock()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            ++serving;
+        }
+        cv.notify_all();
+    }
 };
 
 // Pinned host staging buffer that only grows (async uploads of per-launch
@@ -199,7 +223,7 @@ struct epi_ctx {
     std::string name, arch;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::mutex mu;
+    epi::FairMutex mu;
     // device bytes of pair-margin strips a cohort keeps resident (both
     // classes); a cohort whose full set would exceed it keeps a few strips
     // that its scans refill (epi_set_pair_budget)
@@ -228,6 +252,7 @@ struct epi_ctx {
     hipEvent_t stage_ev[kStageRing] = {};
     bool stage_pending[kStageRing] = {};
     uint32_t stage_next = 0;
+    epi::HostBuf readback;  // pinned destination of a call's results
     // captured batch graph, reused while the batch signature is unchanged
     hipGraphExec_t batch_graph = nullptr;
     std::vector<uint64_t> batch_sig;

@@ -96,7 +96,7 @@ extern "C" int epi_scan_noalloc(epi_ctx *ctx, epi_cohort *co, const epi_tile *ti
     }
     if (!base)
         return EPI_EINTERNAL;  // epi_open takes the address
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     // an MFMA tile needs the cohort's expanded planes, which only an earlier
     // scan with that variant builds; without them the VALU kernel runs (the

@@ -396,8 +396,8 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
     if (timed)
         EPI_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     if (!plan.dump)
-        EPI_HIP(launch_merge(P.hdr, P.slots, counters, plan.cap, (Cand *const *)(base + L.lists),
-                             n, ctx->stream));
+        EPI_HIP(launch_merge(P.hdr, P.slots, counters, P.thr, plan.cap,
+                             (Cand *const *)(base + L.lists), n, ctx->stream));
     return EPI_OK;
 }
 
@@ -493,13 +493,18 @@ int assemble(const Rescored *rs, const epi_cohort *co, epi_result &R, double sca
 static int copy_back_and_assemble(epi_ctx *ctx, Job *const *jobs, uint32_t n, epi_result *results,
                                   std::chrono::steady_clock::time_point t0, bool timed)
 {
-    std::vector<Rescored> host((size_t)n * kTop);
-    uint32_t overflow = 0;
-    EPI_HIP(hipMemcpyAsync(host.data(), ctx->rescored.ptr, sizeof(Rescored) * host.size(),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    EPI_HIP(hipMemcpyAsync(&overflow, (uint32_t *)ctx->counters.ptr + 1 + n, sizeof(uint32_t),
+    // into pinned memory: both copies are plain DMA and one wait covers them
+    // (a copy into pageable memory waits on its own and then copies again)
+    const size_t rb = sizeof(Rescored) * (size_t)n * kTop;
+    EPI_TRY(ctx->readback.reserve(rb + sizeof(uint32_t)));
+    unsigned char *pin = (unsigned char *)ctx->readback.ptr;
+    EPI_HIP(hipMemcpyAsync(pin, ctx->rescored.ptr, rb, hipMemcpyDeviceToHost, ctx->stream));
+    EPI_HIP(hipMemcpyAsync(pin + rb, (uint32_t *)ctx->counters.ptr + 1 + n, sizeof(uint32_t),
                            hipMemcpyDeviceToHost, ctx->stream));
     EPI_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t overflow = 0;
+    std::memcpy(&overflow, pin + rb, sizeof(uint32_t));
+    const Rescored *host = (const Rescored *)pin;
     stage_synced(ctx);
     const double wall = ms_since(t0);
     float kms = 0.0f;
@@ -508,7 +513,7 @@ static int copy_back_and_assemble(epi_ctx *ctx, Job *const *jobs, uint32_t n, ep
     if (overflow != 0)
         return EPI_EINTERNAL;  // a workgroup slot was lost
     for (uint32_t b = 0; b < n; ++b)
-        EPI_TRY(assemble(host.data() + (size_t)b * kTop, jobs[b]->co, results[b], wall, kms));
+        EPI_TRY(assemble(host + (size_t)b * kTop, jobs[b]->co, results[b], wall, kms));
     return EPI_OK;
 }
 
@@ -638,7 +643,7 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
     if (tile && tile->variant > 3)
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     EPI_TRY(ctx_ensure_tables(ctx, cohort->N));
     const ScanConfig cfg = resolve_config(ctx, tile, cohort->N);
@@ -675,7 +680,7 @@ extern "C" int epi_scan_batch(epi_ctx *ctx, epi_cohort *const *cohorts, uint32_t
     EPI_TRY(check_cohorts(ctx, cohorts, n));
     if (tile && tile->variant > 3)
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     uint32_t n_max = 0;
     for (uint32_t b = 0; b < n; ++b) {
@@ -753,7 +758,7 @@ extern "C" int epi_scan_dump(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *t
     EPI_TRY(check_cohorts(ctx, &cohort, 1));
     if (cohort->M > 512 || (tile && tile->variant > 3))
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     EPI_TRY(ctx_ensure_tables(ctx, cohort->N));
     const ScanConfig cfg = resolve_config(ctx, tile, cohort->N);
@@ -798,7 +803,7 @@ extern "C" int epi_count(epi_ctx *ctx, epi_cohort *co, const uint32_t *triplets,
         if (!(i < j && j < k && k < co->M))
             return EPI_EINVAL;
     }
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     EPI_HIP(hipSetDevice(ctx->device));
     EPI_TRY(ctx_ensure_tables(ctx, co->N));
     std::vector<Cand> cands(n);

@@ -372,7 +372,7 @@ void epi_server::run()
             stats.pending = (uint32_t)pending.size();
         }
         if (!cancelled.empty()) {
-            std::lock_guard<std::mutex> g(ctx->mu);
+            std::lock_guard<epi::FairMutex> g(ctx->mu);
             for (Request *r : cancelled)
                 release_locked_ctx(r);  // resident cohorts: drop the reference
         }
@@ -381,7 +381,7 @@ void epi_server::run()
 
         std::vector<Request *> finished, failed;
         {
-            std::lock_guard<std::mutex> g(ctx->mu);
+            std::lock_guard<epi::FairMutex> g(ctx->mu);
             const hipError_t de = hipSetDevice(ctx->device);
             if (de != hipSuccess) {
                 fail_all_locked_ctx(admit, hip_status(de), failed);
@@ -499,7 +499,7 @@ extern "C" int epi_server_create(epi_ctx *ctx, const epi_server_config *config,
     if (!s)
         return EPI_ENOMEM;
     {
-        std::lock_guard<std::mutex> lk(ctx->mu);
+        std::lock_guard<epi::FairMutex> lk(ctx->mu);
         EPI_HIP(hipSetDevice(ctx->device));
         if (cfg.memory_budget == 0) {
             size_t free_b = 0, total_b = 0;
@@ -543,7 +543,7 @@ extern "C" int epi_server_destroy(epi_server *s)
         s->cv_done.wait(lk, [s] { return s->pollers == 0; });
     }
     {
-        std::lock_guard<std::mutex> g(s->ctx->mu);
+        std::lock_guard<epi::FairMutex> g(s->ctx->mu);
         if (s->graph)
             (void)hipGraphExecDestroy(s->graph);
         if (s->upload)
@@ -610,7 +610,7 @@ extern "C" int epi_submit_cohort(epi_server *s, epi_cohort *co, uint64_t *reques
     r->footprint = blocks * blocks * 8 * sizeof(Row) + sizeof(Cand) * kTop;
     r->submitted = Clock::now();
     {
-        std::lock_guard<std::mutex> g(s->ctx->mu);
+        std::lock_guard<epi::FairMutex> g(s->ctx->mu);
         co->server_refs++;
     }
     {
@@ -625,7 +625,7 @@ extern "C" int epi_submit_cohort(epi_server *s, epi_cohort *co, uint64_t *reques
         }
     }
     if (r) {  // not queued: the server is stopping
-        std::lock_guard<std::mutex> g(s->ctx->mu);
+        std::lock_guard<epi::FairMutex> g(s->ctx->mu);
         co->server_refs--;
         return EPI_EBUSY;
     }

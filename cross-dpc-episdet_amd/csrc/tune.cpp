@@ -99,7 +99,7 @@ extern "C" int epi_tune_set(epi_ctx *ctx, const uint32_t *max_samples, const epi
             return EPI_EINVAL;
         t.push_back(x);
     }
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     ctx->tune = t;
     return EPI_OK;
 }
@@ -108,7 +108,7 @@ extern "C" int epi_tune_get(epi_ctx *ctx, uint32_t n_samples, epi_tile *tile)
 {
     if (!ctx || !tile)
         return EPI_EINVAL;
-    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::lock_guard<epi::FairMutex> lk(ctx->mu);
     const ScanConfig c = resolve_config(ctx, nullptr, n_samples);
     std::memset(tile, 0, sizeof(*tile));
     tile->variant = (uint32_t)c.variant;
