@@ -246,6 +246,30 @@ def test_submit_resident_cohort(epi, ctx):
             c.close()
 
 
+def test_plain_scan_beside_busy_server(epi, ctx):
+    """A plain scan on the context of a busy server waits for the server's
+    round in flight, not for its queue to drain: the context lock is taken
+    in arrival order (the worker takes it again after every round)."""
+    g, p = epi.generate_hwe(60, 8192, 1200, 4096)  # about 9 ms of scanning
+    gs, ps = epi.generate_hwe(61, 1000, 60, 500)
+    big, small = ctx.cohort(g, p), ctx.cohort(gs, ps)
+    try:
+        ref = ctx.scan(small)
+        with ctx.server() as srv:
+            ids = [srv.submit_cohort(big) for _ in range(12)]
+            time.sleep(0.005)
+            got = ctx.scan(small)
+            st = srv.stats()
+            say("plain scan beside the server:", st)
+            assert st["completed"] < len(ids)
+            assert same(ref, got)
+            res = [srv.poll(rid, 120000) for rid in ids]
+            assert all(same(res[0], r) for r in res[1:])
+    finally:
+        big.close()
+        small.close()
+
+
 def test_autotune_and_tile_table(epi):
     # a context of its own: the tile table is per context and is replaced here
     with epi.Context(0) as ctx:
