@@ -332,9 +332,81 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant)
 // phenotype other than 1 is a control (GPU/v4.cpp:141); with bed, samples
 // whose phenotype is neither 0 nor 1 are left out, as PLINK leaves out
 // samples with a missing phenotype.
+// The cohort's seed: the 64th best FP32 score (the reference's, from the
+// rescoring kernel, as epi_count gives it) of kSeedSample distinct triplets
+// drawn with a fixed generator, or of every triplet when there are fewer.
+// Any 64 distinct triplets bound the 64th best of all from above, so every
+// scan may start its cohort threshold there: the wavefronts of a short scan
+// then offer and compact only the lanes below it instead of everything
+// they meet before their own lists fill.  Fewer than 64 triplets: no seed.
+constexpr uint32_t kSeedSample = 8192;
+static int cohort_seed(epi_ctx *ctx, epi_cohort *co, hipStream_t strm)
+{
+    const uint64_t M = co->M;
+    const uint64_t total = M < 3 ? 0 : M * (M - 1) * (M - 2) / 6;
+    if (total < (uint64_t)kTop)
+        return EPI_OK;
+    std::vector<uint64_t> keys;
+    if (total <= kSeedSample) {
+        for (uint32_t i = 0; i < M; ++i)
+            for (uint32_t j = i + 1; j < M; ++j)
+                for (uint32_t k = j + 1; k < M; ++k)
+                    keys.push_back(make_key(i, j, k));
+    } else {
+        uint64_t x = 0x9E3779B97F4A7C15ull ^ (M << 32) ^ co->N;  // xorshift64
+        auto draw = [&]() {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            return (uint32_t)((x >> 11) % M);
+        };
+        keys.reserve(kSeedSample);
+        while (keys.size() < kSeedSample) {
+            uint32_t t[3] = {draw(), draw(), draw()};
+            if (t[0] == t[1] || t[0] == t[2] || t[1] == t[2])
+                continue;
+            std::sort(t, t + 3);
+            keys.push_back(make_key(t[0], t[1], t[2]));
+        }
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        if (keys.size() < (size_t)kTop)
+            return EPI_OK;
+    }
+    const uint32_t n = (uint32_t)keys.size();
+    std::vector<Cand> cands(n);
+    for (uint32_t t = 0; t < n; ++t) {
+        cands[t].s = 0.0f;
+        cands[t].pad = 0;
+        cands[t].key = keys[t];
+    }
+    EPI_TRY(ctx_ensure_tables(ctx, co->N));
+    EPI_TRY(ctx->seedc.reserve(sizeof(Cand) * n));
+    EPI_TRY(ctx->seedr.reserve(sizeof(Rescored) * n));
+    EPI_TRY(ctx->readback.reserve(sizeof(Rescored) * n));
+    EPI_HIP(hipMemcpyAsync(ctx->seedc.ptr, cands.data(), sizeof(Cand) * n, hipMemcpyHostToDevice,
+                           strm));
+    EPI_HIP(launch_rescore((const CohortDev *)co->desc.ptr, (const Cand *)ctx->seedc.ptr, nullptr,
+                           n, 1, (const float *)ctx->t32.ptr, (const double *)ctx->t64.ptr,
+                           (Rescored *)ctx->seedr.ptr, strm));
+    EPI_HIP(hipMemcpyAsync(ctx->readback.ptr, ctx->seedr.ptr, sizeof(Rescored) * n,
+                           hipMemcpyDeviceToHost, strm));
+    EPI_HIP(hipStreamSynchronize(strm));
+    const Rescored *rs = (const Rescored *)ctx->readback.ptr;
+    std::vector<float> sc(n);
+    for (uint32_t t = 0; t < n; ++t)
+        sc[t] = rs[t].s32;
+    std::nth_element(sc.begin(), sc.begin() + (kTop - 1), sc.end());
+    co->seed = sc[kTop - 1];
+    const CohortDev h = co->host_desc();
+    EPI_HIP(hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm));
+    EPI_HIP(hipStreamSynchronize(strm));
+    return EPI_OK;
+}
+
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out, hipStream_t strm, bool full_pairs)
+                         epi_cohort **out, hipStream_t strm, bool full_pairs, bool seed)
 {
     // class split in sample order (GPU/v4.cpp:135-171): phenotype 1 = case
     std::vector<uint32_t> order;
@@ -491,6 +563,8 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
         e = hipStreamSynchronize(strm);
     if (st == EPI_OK && e != hipSuccess)
         st = hip_status(e);
+    if (st == EPI_OK && seed)
+        st = cohort_seed(ctx, co, strm);
     if (st != EPI_OK) {
         (void)hipStreamSynchronize(strm);  // nothing in flight may use the blocks
         delete co;
@@ -550,6 +624,7 @@ CohortDev epi_cohort::host_desc() const
     h.pslot = (const uint32_t *)pslot.ptr;
     h.orig = ordered ? (const uint32_t *)perm.ptr : nullptr;
     h.pos = ordered ? (const uint32_t *)perm.ptr + Mp : nullptr;
+    h.seed = seed;
     // class 1's planes as a byte offset from class 0's, when it fits 32 bits
     const size_t e1 = (size_t)Mp * 2 * EL, f1 = (size_t)Mp * 2 * FL;
     h.E1off = e1 < ((size_t)1 << 32) ? (uint32_t)e1 : 0;

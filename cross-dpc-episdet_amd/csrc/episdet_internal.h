@@ -60,6 +60,9 @@ struct CohortDev {
     // is stored at row pos[s].  Both null: rows are in original order.
     const uint32_t *orig;   // [Mp]
     const uint32_t *pos;    // [Mp]
+    // An upper bound of the cohort's 64th best score, set at creation (the
+    // 64th best FP32 score of a sample of distinct triplets; +inf: none).
+    float seed;
 };
 
 // Offset of pair (p, q) in a class's strips, given the slot of q's block.
@@ -172,7 +175,8 @@ hipError_t launch_fill_empty(Cand *p, size_t n, hipStream_t s);
 hipError_t launch_merge(const RoundHdr *hdr, const Cand *slots, const uint32_t *counters,
                         const uint32_t *thr, uint32_t slot_cap, Cand *const *lists,
                         uint32_t max_cohorts, hipStream_t s);
-hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *thr,
+hipError_t launch_init_thr(const RoundHdr *hdr, const CohortDev *cohorts, Cand *const *lists,
+                           uint32_t *thr,
                            uint32_t max_cohorts, hipStream_t s);
 // Rescores per_cohort candidates of each cohort: cands given either as one
 // array [n_cohorts][per_cohort] or (lists != nullptr) as per-cohort lists.

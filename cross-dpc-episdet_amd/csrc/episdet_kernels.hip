@@ -875,24 +875,27 @@ merge_kernel(const RoundHdr *__restrict__ hdr, const Cand *__restrict__ slots,
         list[lane] = part[0][lane];
 }
 
-// Per-launch cohort thresholds from the running lists (64th best score;
-// +inf while a list holds fewer than 64 candidates).
-__global__ void init_thr_kernel(const RoundHdr *__restrict__ hdr, Cand *const *lists,
+// Per-launch cohort thresholds: the running list's 64th best score (+inf
+// while it holds fewer than 64 candidates) or the cohort's seed, whichever
+// is smaller (both bound the 64th best from above).
+__global__ void init_thr_kernel(const RoundHdr *__restrict__ hdr,
+                                const CohortDev *__restrict__ cohorts, Cand *const *lists,
                                 uint32_t *thr, uint32_t max_cohorts)
 {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= max_cohorts)
         return;
-    thr[c] = c < hdr->n_cohorts ? __float_as_uint(lists[c][kTop - 1].s) : 0x7F800000u;
+    thr[c] = c < hdr->n_cohorts ? __float_as_uint(fminf(lists[c][kTop - 1].s, cohorts[c].seed))
+                                : 0x7F800000u;
 }
 
-hipError_t launch_init_thr(const RoundHdr *hdr, Cand *const *lists, uint32_t *thr,
-                           uint32_t max_cohorts, hipStream_t s)
+hipError_t launch_init_thr(const RoundHdr *hdr, const CohortDev *cohorts, Cand *const *lists,
+                           uint32_t *thr, uint32_t max_cohorts, hipStream_t s)
 {
     if (max_cohorts == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(init_thr_kernel, dim3((max_cohorts + 63) / 64), dim3(64), 0, s, hdr, lists,
-                       thr, max_cohorts);
+    hipLaunchKernelGGL(init_thr_kernel, dim3((max_cohorts + 63) / 64), dim3(64), 0, s, hdr,
+                       cohorts, lists, thr, max_cohorts);
     return hipGetLastError();
 }
 

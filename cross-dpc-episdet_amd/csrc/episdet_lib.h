@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <cmath>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -138,10 +139,12 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant);
 // Packs a cohort into device memory on stream `s` (synchronized before
 // returning).  Caller holds ctx->mu.
 // full_pairs: every pair strip resident whatever the context's pair budget
-// (cohorts of the server, whose rounds read any strip).
+// (cohorts of the server, whose rounds read any strip).  seed: sample the
+// cohort threshold's seed (cohort_seed).
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
-                         epi_cohort **out, hipStream_t s, bool full_pairs = false);
+                         epi_cohort **out, hipStream_t s, bool full_pairs = false,
+                         bool seed = true);
 void build_log_tables(uint32_t n_max, std::vector<float> &t32, std::vector<double> &t64);
 int ctx_ensure_tables(epi_ctx *ctx, uint32_t n_samples);
 ScanConfig resolve_config(const epi_ctx *ctx, const epi_tile *tile, uint32_t n_samples);
@@ -253,6 +256,7 @@ struct epi_ctx {
     bool stage_pending[kStageRing] = {};
     uint32_t stage_next = 0;
     epi::HostBuf readback;  // pinned destination of a call's results
+    epi::DevBuf seedc, seedr;  // a cohort creation's sample triplets and their scores
     // captured batch graph, reused while the batch signature is unchanged
     hipGraphExec_t batch_graph = nullptr;
     std::vector<uint64_t> batch_sig;
@@ -283,6 +287,9 @@ struct epi_cohort {
     // then pos[Mp] (original SNP -> row)
     epi::DevBuf perm;
     bool ordered = false;
+    // the 64th best FP32 score of a sample of distinct triplets (an upper
+    // bound of the cohort's 64th best; +inf when none was taken)
+    float seed = INFINITY;
     // triplet-tile rows for tile width rows_jp (0: none yet), built on first use
     std::vector<epi::Row> rows;
     uint64_t rows_items = 0;

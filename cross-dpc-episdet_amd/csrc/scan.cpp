@@ -388,7 +388,8 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
     P.overflow = counters + 1 + n;
     P.dump_scores = dump_scores;
     P.dump_counts = dump_counts;
-    EPI_HIP(launch_init_thr(P.hdr, (Cand *const *)(base + L.lists), P.thr, n, ctx->stream));
+    EPI_HIP(launch_init_thr(P.hdr, P.cohorts, (Cand *const *)(base + L.lists), P.thr, n,
+                            ctx->stream));
     if (timed)
         EPI_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (any_items)

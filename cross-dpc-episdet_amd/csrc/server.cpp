@@ -150,8 +150,10 @@ int epi_server::admit_locked_ctx(Request *r)
 {
     int st = EPI_OK;
     if (!r->resident)
+        // no seed: sampling it costs an admission more than it saves the
+        // one scan of such a cohort (the next round waits for admissions)
         st = cohort_create_locked(ctx, r->geno.data(), nullptr, r->pheno.data(), r->N, r->M,
-                                  &r->co, upload, true);
+                                  &r->co, upload, true, false);
     if (st == EPI_OK)
         st = job_init(ctx, r->job, r->co, scfg, nullptr);
     if (st == EPI_OK && r->resident) {

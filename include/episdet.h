@@ -107,7 +107,10 @@ int epi_device_name(epi_ctx *ctx, char *buf, size_t len);
 /* Cohorts -------------------------------------------------------------- */
 /* Class split and bit-packing into resident device planes.  Replaces
  * data_to_binary + binary_to_tiled + the USM uploads (GPU/v4.cpp:112-196,
- * :220-230, :256-274).  n_snps >= 3, n_samples >= 1. */
+ * :220-230, :256-274).  n_snps >= 3, n_samples >= 1.  Also scores 8192
+ * distinct triplets (all of them, when there are fewer) and keeps the 64th
+ * best score as the starting threshold of every scan of the cohort; scans
+ * give the same results either way. */
 int epi_cohort_create(epi_ctx *ctx, const uint8_t *geno, const uint8_t *pheno,
                       uint32_t n_samples, uint32_t n_snps, epi_cohort **out);
 /* Waits for the context's queued work, then gives the cohort's device
