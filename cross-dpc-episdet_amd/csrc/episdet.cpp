@@ -515,10 +515,11 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     // packed again in that order; results map back to original indices.
     // Lanes that may reach the top list and whose SNPs are out of original
     // order are summed twice, which costs more than the order saves until
-    // each wavefront's threshold has settled: below kOrderMinSnps SNPs the
-    // scan is too short for that, and below kOrderMinSamples samples the
-    // counts are small and the lookups conflict little in either order (by
-    // default the order is used from there).
+    // each wavefront's threshold has settled (the cohort's seed settles it
+    // early): below kOrderMinSnps SNPs the scan is too short for the order
+    // to pay, and below kOrderMinSamples samples the counts are small and
+    // the lookups conflict little in either order (by default the order is
+    // used from there).
     std::vector<uint32_t> perm;
     const bool ordered =
         ctx->snp_order == 1 ||

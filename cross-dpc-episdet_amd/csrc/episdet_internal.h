@@ -13,7 +13,7 @@ constexpr int kWaveCap = 128;  // per-wavefront LDS candidate buffer
 constexpr int kMfmaWaves = 8;  // wavefronts per workgroup of the MFMA scan kernel (2 per SIMD)
 constexpr uint32_t kMaxSamples = 1u << 22;    // n_samples below this (exact FP4 sums)
 constexpr uint32_t kMaxSnps = (1u << 21) - 1;  // n_snps at most this (21-bit key fields)
-constexpr uint32_t kOrderMinSnps = 1024;     // allele-frequency SNP order by default from this many
+constexpr uint32_t kOrderMinSnps = 512;      // allele-frequency SNP order by default from this many
 constexpr uint32_t kOrderMinSamples = 2048;  // SNPs and samples on
 
 // Candidate: FP32 score and a lexicographic key (i << 42 | j << 21 | k).

@@ -365,7 +365,7 @@ class Context:
 
     def set_snp_order(self, mode: int) -> None:
         """Device SNP order of cohorts created afterwards: 1 allele
-        frequency, 0 original, 2 (the default) allele frequency from 1024
+        frequency, 0 original, 2 (the default) allele frequency from 512
         SNPs and 2048 samples on.  Results are those of the original order either way."""
         _check(lib().epi_set_snp_order(self.handle, int(mode)), "epi_set_snp_order")
 

@@ -140,7 +140,7 @@ int epi_set_pair_budget(epi_ctx *ctx, uint64_t bytes);
 /* Device SNP order of cohorts created after this call: 1 allele
  * frequency, which makes the scan's ln(n!) lookups cheaper on long scans;
  * 0 the original order; 2 (the default) allele frequency for cohorts of
- * at least 1024 SNPs and 2048 samples, else original.  Every result (indices, scores,
+ * at least 512 SNPs and 2048 samples, else original.  Every result (indices, scores,
  * counts, tie order, dumps) is that of the original order bit for bit
  * either way.  The reference keeps the input order (GPU/v4.cpp:112-196);
  * no counterpart. */
