@@ -15,12 +15,14 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
 #include "../../include/episdet.h"
 #include "episdet_internal.h"
 #include "episdet_lib.h"
+#include "episdet_sample.h"
 
 using namespace epi;
 
@@ -327,83 +329,58 @@ uint64_t cohort_footprint(uint32_t n_samples, uint32_t n_snps, int variant)
            sizeof(CohortDev) + 4096;
 }
 
+// The cohort's seed (episdet_sample.h): the 64th best FP32 score (the
+// reference's, from the rescoring kernel, as epi_count gives it) of the
+// sample triplets.  Every scan may start its cohort threshold there: the
+// wavefronts of a short scan then offer and compact only the lanes below it
+// instead of everything they meet before their own lists fill.  The
+// triplets are unranked on the host, scored and the 64th best selected on
+// the device, and the selected score is copied back on `strm`, which the
+// caller synchronizes before cohort_seed_take.  Fewer than 64 triplets: no
+// seed.
+static int cohort_seed_enqueue(epi_ctx *ctx, epi_cohort *co, hipStream_t strm, bool &pending)
+{
+    pending = false;
+    const uint64_t total = choose3(co->M);
+    if (total < (uint64_t)kTop)
+        return EPI_OK;
+    const uint32_t n = (uint32_t)std::min<uint64_t>(total, kSeedSample);
+    EPI_TRY(ctx_ensure_tables(ctx, co->N));
+    EPI_TRY(ctx->seedc.reserve(sizeof(Cand) * kSeedSample));
+    EPI_TRY(ctx->seedr.reserve(sizeof(Rescored) * kSeedSample + 256));
+    EPI_TRY(ctx->seedh.reserve(sizeof(Cand) * kSeedSample + 256));
+    Cand *hc = (Cand *)ctx->seedh.ptr;
+    const SeedSampler smp(total, co->N);
+    for (uint32_t t = 0; t < n; ++t) {
+        uint32_t x[3];
+        colex_triplet(smp.rank(t), co->M, x);
+        hc[t].s = 0.0f;
+        hc[t].pad = 0;
+        hc[t].key = make_key(x[0], x[1], x[2]);
+    }
+    float *dsel = (float *)((unsigned char *)ctx->seedr.ptr + sizeof(Rescored) * kSeedSample);
+    float *hsel = (float *)((unsigned char *)ctx->seedh.ptr + sizeof(Cand) * kSeedSample);
+    EPI_HIP(hipMemcpyAsync(ctx->seedc.ptr, hc, sizeof(Cand) * n, hipMemcpyHostToDevice, strm));
+    EPI_HIP(launch_rescore((const CohortDev *)co->desc.ptr, (const Cand *)ctx->seedc.ptr, nullptr,
+                           n, 1, (const float *)ctx->t32.ptr, (const double *)ctx->t64.ptr,
+                           (Rescored *)ctx->seedr.ptr, strm));
+    EPI_HIP(launch_seed_select((const Rescored *)ctx->seedr.ptr, n, dsel, strm));
+    EPI_HIP(hipMemcpyAsync(hsel, dsel, sizeof(float), hipMemcpyDeviceToHost, strm));
+    pending = true;
+    return EPI_OK;
+}
+
+// After `strm` has been synchronized: the selected score.
+static void cohort_seed_take(const epi_ctx *ctx, epi_cohort *co)
+{
+    co->seed = *(const float *)((const unsigned char *)ctx->seedh.ptr + sizeof(Cand) * kSeedSample);
+}
+
 // Caller holds ctx->mu.  Exactly one of geno (sample-major bytes) and bed
 // (PLINK 1 .bed body, unpacked on the device) is given.  With geno, any
 // phenotype other than 1 is a control (GPU/v4.cpp:141); with bed, samples
 // whose phenotype is neither 0 nor 1 are left out, as PLINK leaves out
 // samples with a missing phenotype.
-// The cohort's seed: the 64th best FP32 score (the reference's, from the
-// rescoring kernel, as epi_count gives it) of kSeedSample distinct triplets
-// drawn with a fixed generator, or of every triplet when there are fewer.
-// Any 64 distinct triplets bound the 64th best of all from above, so every
-// scan may start its cohort threshold there: the wavefronts of a short scan
-// then offer and compact only the lanes below it instead of everything
-// they meet before their own lists fill.  Fewer than 64 triplets: no seed.
-constexpr uint32_t kSeedSample = 8192;
-static int cohort_seed(epi_ctx *ctx, epi_cohort *co, hipStream_t strm)
-{
-    const uint64_t M = co->M;
-    const uint64_t total = M < 3 ? 0 : M * (M - 1) * (M - 2) / 6;
-    if (total < (uint64_t)kTop)
-        return EPI_OK;
-    std::vector<uint64_t> keys;
-    if (total <= kSeedSample) {
-        for (uint32_t i = 0; i < M; ++i)
-            for (uint32_t j = i + 1; j < M; ++j)
-                for (uint32_t k = j + 1; k < M; ++k)
-                    keys.push_back(make_key(i, j, k));
-    } else {
-        uint64_t x = 0x9E3779B97F4A7C15ull ^ (M << 32) ^ co->N;  // xorshift64
-        auto draw = [&]() {
-            x ^= x << 13;
-            x ^= x >> 7;
-            x ^= x << 17;
-            return (uint32_t)((x >> 11) % M);
-        };
-        keys.reserve(kSeedSample);
-        while (keys.size() < kSeedSample) {
-            uint32_t t[3] = {draw(), draw(), draw()};
-            if (t[0] == t[1] || t[0] == t[2] || t[1] == t[2])
-                continue;
-            std::sort(t, t + 3);
-            keys.push_back(make_key(t[0], t[1], t[2]));
-        }
-        std::sort(keys.begin(), keys.end());
-        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-        if (keys.size() < (size_t)kTop)
-            return EPI_OK;
-    }
-    const uint32_t n = (uint32_t)keys.size();
-    std::vector<Cand> cands(n);
-    for (uint32_t t = 0; t < n; ++t) {
-        cands[t].s = 0.0f;
-        cands[t].pad = 0;
-        cands[t].key = keys[t];
-    }
-    EPI_TRY(ctx_ensure_tables(ctx, co->N));
-    EPI_TRY(ctx->seedc.reserve(sizeof(Cand) * n));
-    EPI_TRY(ctx->seedr.reserve(sizeof(Rescored) * n));
-    EPI_TRY(ctx->readback.reserve(sizeof(Rescored) * n));
-    EPI_HIP(hipMemcpyAsync(ctx->seedc.ptr, cands.data(), sizeof(Cand) * n, hipMemcpyHostToDevice,
-                           strm));
-    EPI_HIP(launch_rescore((const CohortDev *)co->desc.ptr, (const Cand *)ctx->seedc.ptr, nullptr,
-                           n, 1, (const float *)ctx->t32.ptr, (const double *)ctx->t64.ptr,
-                           (Rescored *)ctx->seedr.ptr, strm));
-    EPI_HIP(hipMemcpyAsync(ctx->readback.ptr, ctx->seedr.ptr, sizeof(Rescored) * n,
-                           hipMemcpyDeviceToHost, strm));
-    EPI_HIP(hipStreamSynchronize(strm));
-    const Rescored *rs = (const Rescored *)ctx->readback.ptr;
-    std::vector<float> sc(n);
-    for (uint32_t t = 0; t < n; ++t)
-        sc[t] = rs[t].s32;
-    std::nth_element(sc.begin(), sc.begin() + (kTop - 1), sc.end());
-    co->seed = sc[kTop - 1];
-    const CohortDev h = co->host_desc();
-    EPI_HIP(hipMemcpyAsync(co->desc.ptr, &h, sizeof(h), hipMemcpyHostToDevice, strm));
-    EPI_HIP(hipStreamSynchronize(strm));
-    return EPI_OK;
-}
-
 int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
                          const uint8_t *pheno, uint32_t n_samples, uint32_t n_snps,
                          epi_cohort **out, hipStream_t strm, bool full_pairs, bool seed)
@@ -560,12 +537,15 @@ int cohort_create_locked(epi_ctx *ctx, const uint8_t *geno, const uint8_t *bed,
     }
     if (st == EPI_OK && e == hipSuccess && !co->blocked)
         e = launch_strips((const CohortDev *)co->desc.ptr, nullptr, nb, co->Mp, strm);
+    bool seeding = false;
+    if (st == EPI_OK && e == hipSuccess && seed)
+        st = cohort_seed_enqueue(ctx, co, strm, seeding);
     if (st == EPI_OK && e == hipSuccess)
         e = hipStreamSynchronize(strm);
     if (st == EPI_OK && e != hipSuccess)
         st = hip_status(e);
-    if (st == EPI_OK && seed)
-        st = cohort_seed(ctx, co, strm);
+    if (st == EPI_OK && seeding)
+        cohort_seed_take(ctx, co);
     if (st != EPI_OK) {
         (void)hipStreamSynchronize(strm);  // nothing in flight may use the blocks
         delete co;

@@ -180,6 +180,8 @@ hipError_t launch_init_thr(const RoundHdr *hdr, const CohortDev *cohorts, Cand *
                            uint32_t max_cohorts, hipStream_t s);
 // Rescores per_cohort candidates of each cohort: cands given either as one
 // array [n_cohorts][per_cohort] or (lists != nullptr) as per-cohort lists.
+// The kTop-th smallest s32 of rs[0..n) (kTop <= n <= 8192) into *out.
+hipError_t launch_seed_select(const Rescored *rs, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands,
                           Cand *const *lists, uint32_t per_cohort, uint32_t n_cohorts,
                           const float *t32, const double *t64, Rescored *out,

@@ -999,6 +999,60 @@ rescore_kernel(const CohortDev *__restrict__ cohorts, const Cand *__restrict__ c
     }
 }
 
+// The kTop-th smallest FP32 score of n rescored triplets (kTop <= n <=
+// 8 x 1024) into *out: a binary search over the score bits (scores are >= 0,
+// so their bits order like the values) with a block-wide count per step,
+// one LDS add per wavefront.  The three counters rotate: thread 0 clears the
+// one the previous step read (every wavefront has read it before this
+// step's barrier), which is next added to two steps later, after the next
+// barrier.
+__global__ void __launch_bounds__(1024) seed_select_kernel(const Rescored *__restrict__ rs,
+                                                           uint32_t n, float *out)
+{
+    __shared__ uint32_t cnt[3];
+    const uint32_t t = threadIdx.x;
+    uint32_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t x = t + 1024u * q;
+        v[q] = x < n ? __float_as_uint(rs[x].s32) : 0xFFFFFFFFu;
+    }
+    if (t < 3)
+        cnt[t] = 0;
+    __syncthreads();
+    uint32_t lo = 0, hi = 0x7F800000u;  // at least kTop scores are <= hi
+    for (uint32_t it = 0; lo < hi; ++it) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            c += v[q] <= mid;
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1)
+            c += (uint32_t)__shfl_xor((int)c, m);
+        if ((t & 63u) == 0)
+            atomicAdd(&cnt[it % 3], c);
+        __syncthreads();
+        const uint32_t tot = cnt[it % 3];
+        if (t == 0)
+            cnt[(it + 2) % 3] = 0;
+        if (tot >= (uint32_t)kTop)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    if (t == 0)
+        *out = __uint_as_float(lo);
+}
+
+hipError_t launch_seed_select(const Rescored *rs, uint32_t n, float *out, hipStream_t s)
+{
+    if (n < (uint32_t)kTop || n > 8192)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(seed_select_kernel, dim3(1), dim3(1024), 0, s, rs, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_rescore(const CohortDev *cohorts, const Cand *cands, Cand *const *lists,
                           uint32_t per_cohort, uint32_t n_cohorts, const float *t32,
                           const double *t64, Rescored *out, hipStream_t s)

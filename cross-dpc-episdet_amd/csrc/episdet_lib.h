@@ -256,7 +256,10 @@ struct epi_ctx {
     bool stage_pending[kStageRing] = {};
     uint32_t stage_next = 0;
     epi::HostBuf readback;  // pinned destination of a call's results
-    epi::DevBuf seedc, seedr;  // a cohort creation's sample triplets and their scores
+    // a cohort creation's seed sample: triplets and scores on the device,
+    // the triplets and the selected score in pinned host memory
+    epi::DevBuf seedc, seedr;
+    epi::HostBuf seedh;
     // captured batch graph, reused while the batch signature is unchanged
     hipGraphExec_t batch_graph = nullptr;
     std::vector<uint64_t> batch_sig;

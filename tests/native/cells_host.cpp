@@ -6,6 +6,7 @@
 
 #include "../../cross-dpc-episdet_amd/csrc/episdet_cells.h"
 #include "../../cross-dpc-episdet_amd/csrc/episdet_order.h"
+#include "../../cross-dpc-episdet_amd/csrc/episdet_sample.h"
 #include "../../include/episdet.h"
 
 extern "C" void cells_derive(const uint32_t *t8, const uint32_t *ij, const uint32_t *ik,
@@ -52,4 +53,12 @@ extern "C" void order_ref_derive(uint32_t o, const uint32_t *t8, const uint32_t 
     }
     epi::to_ref_axes(o, st, rf);
     epi::derive27(rf.t8, rf.ij, rf.ik, rf.jk, rf.si, rf.sj, rf.sk, n, x);
+}
+
+// the seed sample (episdet_sample.h): triplet of colex rank r, t-th rank
+extern "C" void sample_triplet(uint64_t r, uint32_t M, uint32_t *out) { epi::colex_triplet(r, M, out); }
+
+extern "C" uint64_t sample_rank(uint64_t total, uint32_t N, uint32_t t)
+{
+    return epi::SeedSampler(total, N).rank(t);
 }

@@ -101,3 +101,58 @@ def test_reference_axes_from_stored_axes(order_lib, n, seed):
         assert np.array_equal(x, ref), (orig, o)
         sc = [order_lib.order_stored_cell(o, m) for m in range(27)]
         assert np.array_equal(stored[sc], ref), (orig, o)
+
+
+# ---------------------------------------------------------------------------
+# The triplet sample of a cohort's threshold seed (episdet_sample.h): its 64th
+# best score bounds the cohort's 64th best only if the triplets are distinct.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def sample_lib():
+    if not os.path.exists(SO):
+        pytest.skip("tests/native/libcells_host.so not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(SO)
+    L.sample_triplet.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    L.sample_rank.restype = ctypes.c_uint64
+    L.sample_rank.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+    return L
+
+
+def _triplet(L, r, M):
+    out = (ctypes.c_uint32 * 3)()
+    L.sample_triplet(r, M, out)
+    return tuple(out)
+
+
+def test_colex_unranking_enumerates_every_triplet_once(sample_lib):
+    import itertools
+    from math import comb
+    for M in (3, 4, 5, 7, 16, 33, 64, 65):
+        got = [_triplet(sample_lib, r, M) for r in range(comb(M, 3))]
+        assert sorted(got) == list(itertools.combinations(range(M), 3)), M
+        # colex order: the last index leads
+        assert got == sorted(got, key=lambda t: (t[2], t[1], t[0])), M
+
+
+def test_colex_unranking_inverts_the_rank_at_scale(sample_lib):
+    from math import comb
+    rng = np.random.default_rng(9)
+    for M in (4096, 100000, (1 << 21) - 1):
+        total = comb(M, 3)
+        for r in [0, 1, total - 1] + [int(x) for x in rng.integers(0, total, 200)]:
+            i, j, k = _triplet(sample_lib, r, M)
+            assert 0 <= i < j < k < M
+            assert comb(k, 3) + comb(j, 2) + i == r
+
+
+def test_sample_ranks_are_distinct(sample_lib):
+    from math import comb
+    for M, N in ((10, 100), (40, 1000), (41, 1000), (300, 2048), (4096, 16384),
+                 ((1 << 21) - 1, 77)):
+        total = comb(M, 3)
+        n = min(total, 8192)
+        ranks = [sample_lib.sample_rank(total, N, t) for t in range(n)]
+        assert all(0 <= r < total for r in ranks)
+        assert len(set(ranks)) == n, (M, N)
+        if total <= 8192:
+            assert ranks == list(range(total))
