@@ -187,3 +187,23 @@ def test_bed_cohort_in_stored_order(epi, orc, ord_ctx, plain_ctx):
         ca, sa, _ = plain_ctx.count(a, tr)
     assert same(ra, rb)
     assert np.array_equal(ca, cb) and np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("var", VARIANTS)
+def test_seed_of_every_triplet_with_ties(epi, orc, plain_ctx, var):
+    """A cohort of 24 SNPs has 2024 triplets, so its threshold seed is the
+    exact 64th best score of all of them; three copies of 8 SNPs make many
+    triplets tie with it.  The top 64 (scores, ties in lexicographic order)
+    equal the oracle's."""
+    g8, p = epi.generate_hwe(38, 900, 8, 450)
+    g = np.ascontiguousarray(np.concatenate([g8, g8, g8], 1))
+    say("seed ties", var)
+    with plain_ctx.cohort(g, p) as co:
+        a = plain_ctx.scan(co, tile={"variant": var})
+    _, top = orc.PackedCohort(g, p).scan(topk=64)
+    assert len(a.top) == 64
+    assert len(set(np.float32(t[0]).tobytes() for t in top)) < 64  # ties present
+    for e, c in enumerate(a.top):
+        s_, i, j, k = top[e]
+        assert (c.i, c.j, c.k) == (i, j, k), e
+        assert np.float32(c.score).tobytes() == np.float32(s_).tobytes(), e
