@@ -195,9 +195,10 @@ int reserve_lists(epi_ctx *ctx, uint32_t n);
 int ensure_expanded(epi_ctx *ctx, epi_cohort *co, int variant);
 // Uploads the round tables (rows [ranges[b].first, ranges[b].second) of
 // jobs[b]) and enqueues queue reset, scan kernel and merge.
+// layout (may be null): where the round's tables sit in ctx->round.
 int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
                   const std::pair<size_t, size_t> *ranges, uint32_t n, bool timed,
-                  float *dump_scores, uint32_t *dump_counts);
+                  float *dump_scores, uint32_t *dump_counts, RoundLayout *layout = nullptr);
 // Enqueues the rescoring of the jobs' running lists and the copy of the
 // rescored candidates into `out` (pinned, grown as needed); once the stream
 // has passed that copy, assemble_finished turns them into epi_results, with

@@ -404,7 +404,7 @@ int enqueue_round_kernels(epi_ctx *ctx, const LaunchPlan &plan, uint32_t n, cons
 
 int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
                   const std::pair<size_t, size_t> *ranges, uint32_t n, bool timed,
-                  float *dump_scores, uint32_t *dump_counts)
+                  float *dump_scores, uint32_t *dump_counts, RoundLayout *layout)
 {
     std::vector<unsigned char> host;
     RoundLayout L;
@@ -412,6 +412,8 @@ int enqueue_round(epi_ctx *ctx, const LaunchPlan &plan, Job *const *jobs,
     EPI_TRY(build_round(jobs, ranges, n, n, host, L, items));
     EPI_TRY(reserve_round(ctx, plan, n, L.total));
     EPI_TRY(stage_upload(ctx, ctx->round.ptr, host.data(), L.total));
+    if (layout)
+        *layout = L;
     return enqueue_round_kernels(ctx, plan, n, L, ctx_round_bufs(ctx), items > 0, timed,
                                  dump_scores, dump_counts);
 }
@@ -664,8 +666,16 @@ extern "C" int epi_scan(epi_ctx *ctx, epi_cohort *cohort, const epi_tile *tile,
         return EPI_OK;
     }
     const std::pair<size_t, size_t> all(0, job.rows.size());
-    EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, true, nullptr, nullptr));
-    EPI_TRY(enqueue_rescore(ctx, jp, 1));
+    RoundLayout L;
+    EPI_TRY(enqueue_round(ctx, plan, jp, &all, 1, true, nullptr, nullptr, &L));
+    // the rescoring reads the round's own tables (the cohort's descriptor
+    // and its list pointer), which the scan left on the device
+    EPI_TRY(ctx->rescored.reserve(sizeof(Rescored) * kTop));
+    const unsigned char *rb = (const unsigned char *)ctx->round.ptr;
+    EPI_HIP(launch_rescore((const CohortDev *)(rb + L.descs), nullptr,
+                           (Cand *const *)(rb + L.lists), kTop, 1, (const float *)ctx->t32.ptr,
+                           (const double *)ctx->t64.ptr, (Rescored *)ctx->rescored.ptr,
+                           ctx->stream));
     return copy_back_and_assemble(ctx, jp, 1, result, t0, true);
 }
 
